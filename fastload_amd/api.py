@@ -1,0 +1,231 @@
+"""Python mirror of the reference host API (same names, argument meaning and error behaviour as
+src/auxiliary.h, src/csr2csc_cuda.h, src/ColSort_GPU.h, src/formatTransform_GPU.h,
+src/FastLoad_GPU.h), over the C ABI of libfastload.so, plus the device-resident FastLoadMatrix.
+
+The reference returns results through pointer/reference parameters and prints errors; here the
+outputs are returned as numpy arrays and every failure raises FastLoadError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import FastLoadError, MatrixInfo, SlideMatrix, check, i32p, f64p, lib
+from .device import DeviceArray
+
+VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2 = 0, 1, 2, 3
+KERNEL_DROPIN, KERNEL_FAST = 0, 1
+
+GEN_SYMGEOM, GEN_ZIPF, GEN_STENCIL27, GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF_DENSE = range(7)
+VEC_NORMAL, VEC_UNIFORM, VEC_MOD10 = range(3)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _copy_out(ptr, count, dtype):
+    if count <= 0 or not ptr:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+
+# ------------------------------------------------------------------ reference host API
+def mmio_allinone(filename: str):
+    """src/auxiliary.h:65-232 -> (m, n, nnz, isSymmetric, rowptr, colidx, val)."""
+    m, n, nnz, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    rp, ci, v = i32p(), i32p(), f64p()
+    check(lib.fl_mmio_allinone(C.byref(m), C.byref(n), C.byref(nnz), C.byref(s), C.byref(rp), C.byref(ci), C.byref(v),
+                               str(filename).encode()), "mmio_allinone")
+    out = (m.value, n.value, nnz.value, s.value, _copy_out(rp, m.value + 1, np.int32),
+           _copy_out(ci, nnz.value, np.int32), _copy_out(v, nnz.value, np.float64))
+    for p in (rp, ci, v):
+        lib.fl_free(C.cast(p, C.c_void_p))
+    return out
+
+
+def csr_to_csc(m: int, n: int, csr_val, csr_row_ptr, csr_col_ind):
+    """src/csr2csc_cuda.h:51-135 -> (csc_val, csc_row_ind, csc_col_ptr, nnzpercol, time_ms)."""
+    rp = np.ascontiguousarray(csr_row_ptr, np.int32)
+    nnz = int(rp[m])
+    ci = np.ascontiguousarray(csr_col_ind, np.int32)
+    v = np.ascontiguousarray(csr_val, np.float64)
+    cv = np.zeros(max(nnz, 1)); cr = np.zeros(max(nnz, 1), np.int32)
+    cp = np.zeros(n + 1, np.int32); npc = np.zeros(max(n, 1), np.int32)
+    t = C.c_double()
+    check(lib.fl_csr_to_csc(C.byref(t), m, n, nnz, _p(v), _p(rp), _p(ci), _p(cv), _p(cr), _p(cp), _p(npc)), "csr_to_csc")
+    return cv[:nnz], cr[:nnz], cp, npc[:n], t.value
+
+
+def ColSort(rowA: int, colA: int, nnzPerCol, cscColPtr, cscRowIdx, cscVal, x):
+    """src/ColSort_GPU.h:46-137 -> (sortrowidxTmp, sortvalTmp, sortnnzTmp, sortx, time_ms)."""
+    cp = np.ascontiguousarray(cscColPtr, np.int32)
+    nnz = int(cp[colA])
+    npc = np.ascontiguousarray(nnzPerCol, np.int32)
+    ri = np.ascontiguousarray(cscRowIdx, np.int32)
+    cv = np.ascontiguousarray(cscVal, np.float64)
+    xx = np.ascontiguousarray(x, np.float64)
+    tr = np.zeros(max(nnz, 1), np.int32); tv = np.zeros(max(nnz, 1))
+    sn = np.zeros(max(colA, 1), np.int32); sx = np.zeros(max(colA, 1))
+    t = C.c_double()
+    check(lib.fl_ColSort(C.byref(t), rowA, colA, nnz, _p(npc), _p(cp), _p(ri), _p(cv), _p(tr), _p(tv), _p(sn),
+                         _p(xx) if colA else _p(sx), _p(sx)), "ColSort")
+    return tr[:nnz], tv[:nnz], sn[:colA], sx[:colA], t.value
+
+
+@dataclass
+class Tiles:
+    """Host copy of a slide_matrix (src/format.h:6-20)."""
+    tilenum: int
+    segsum: int
+    tile_ptr: np.ndarray
+    tile_len: np.ndarray
+    tile_colidx: np.ndarray
+    tile_format: np.ndarray
+    sortrowidx: np.ndarray
+    sortval: np.ndarray
+    sortrowindex: np.ndarray
+    countrow: np.ndarray
+    segmentoffset: np.ndarray
+
+
+def formatTransform(sortrowidx_tmp, sortval_tmp, sortnnz_tmp, nnz: int, rowA: int, colA: int):
+    """src/formatTransform_GPU.h:289-596 -> (Tiles, h_count, timeFormatTran, timeFormatclas)."""
+    sm = SlideMatrix()
+    tr = np.ascontiguousarray(sortrowidx_tmp, np.int32)
+    tv = np.ascontiguousarray(sortval_tmp, np.float64)
+    sn = np.ascontiguousarray(sortnnz_tmp, np.int32)
+    a, b, h = C.c_double(), C.c_double(), C.c_int()
+    check(lib.fl_formatTransform(C.byref(a), C.byref(b), C.byref(sm), _p(tr) if nnz else None,
+                                 _p(tv) if nnz else None, _p(sn) if colA else _p(np.zeros(1, np.int32)), nnz, rowA,
+                                 colA, C.byref(h)), "formatTransform")
+    try:
+        T, S = sm.tilenum, sm.segsum
+        tiles = Tiles(T, S, _copy_out(sm.tile_ptr, T + 1, np.int32), _copy_out(sm.tile_len, T, np.int32),
+                      _copy_out(sm.tile_colidx, T, np.int32), _copy_out(sm.tile_format, T, np.int32),
+                      _copy_out(sm.sortrowidx, nnz, np.int32), _copy_out(sm.sortval, nnz, np.float64),
+                      _copy_out(sm.sortrowindex, nnz, np.int32), _copy_out(sm.countrow, T + 1, np.int32),
+                      _copy_out(sm.segmentoffset, S + 1, np.int32))
+    finally:
+        lib.fl_Tile_destroy(C.byref(sm))
+    return tiles, h.value, a.value, b.value
+
+
+def _slide(t: Tiles):
+    keep = dict(tile_ptr=np.ascontiguousarray(t.tile_ptr, np.int32), tile_len=np.ascontiguousarray(t.tile_len, np.int32),
+                tile_colidx=np.ascontiguousarray(t.tile_colidx, np.int32),
+                tile_format=np.ascontiguousarray(t.tile_format, np.int32),
+                sortrowidx=np.ascontiguousarray(t.sortrowidx, np.int32),
+                sortval=np.ascontiguousarray(t.sortval, np.float64),
+                sortrowindex=np.ascontiguousarray(t.sortrowindex, np.int32))
+    sm = SlideMatrix()
+    for k, a in keep.items():
+        setattr(sm, k, C.cast(a.ctypes.data, f64p if a.dtype == np.float64 else i32p))
+    sm.tilenum = t.tilenum
+    sm.segsum = t.segsum
+    return sm, keep
+
+
+def FastLoad_spmv(tiles: Tiles, nnz: int, colA: int, rowA: int, x, y_golden=None, filename: str | None = None):
+    """src/FastLoad_GPU.h:275-409 (note the colA, rowA order) -> (y, mean_ms, passed)."""
+    sm, keep = _slide(tiles)
+    xx = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(max(rowA, 1))
+    g = None if y_golden is None else np.ascontiguousarray(y_golden, np.float64)
+    t = C.c_double()
+    rc = check(lib.fl_FastLoad_spmv(filename.encode() if filename else None, C.byref(sm), nnz, colA, rowA,
+                                    _p(xx) if colA else _p(np.zeros(1)), _p(y), _p(g), C.byref(t)), "FastLoad_spmv")
+    del keep
+    return y[:rowA], t.value, rc == 0
+
+
+# ---------------------------------------------------------------- device-resident matrix
+class FastLoadMatrix:
+    """ColSort + formatTransform run on the GPU and kept resident (include/fastload.h)."""
+
+    def __init__(self, m: int, n: int, colptr, rowidx, val, width: int = 32, stream=None):
+        if isinstance(colptr, DeviceArray):
+            dcp, dri, dv = colptr, rowidx, val
+        else:
+            dcp = DeviceArray.from_host(np.ascontiguousarray(colptr, np.int32))
+            dri = DeviceArray.from_host(np.ascontiguousarray(rowidx, np.int32))
+            dv = DeviceArray.from_host(np.ascontiguousarray(val, np.float64))
+        nnz = dri.count if isinstance(rowidx, DeviceArray) else int(np.asarray(colptr)[n])
+        h = C.c_void_p()
+        check(lib.fl_matrix_create(C.byref(h), m, n, nnz, dcp.ptr, dri.ptr, dv.ptr, width,
+                                   stream.ptr if stream is not None else None), "fl_matrix_create")
+        self.handle = h.value
+        info = MatrixInfo()
+        check(lib.fl_matrix_get_info(self.handle, C.byref(info)), "fl_matrix_get_info")
+        self.info = {k: getattr(info, k) for k, _ in MatrixInfo._fields_}
+        self.m, self.n, self.nnz, self.width = m, n, nnz, width
+        self.tilenum, self.levels = self.info["tilenum"], self.info["levels"]
+
+    def export(self) -> dict:
+        i = self.info
+        n, T, nz, L, S = i["n"], i["tilenum"], i["nnz"], i["levels"], i["segsum"]
+        out = dict(perm=np.zeros(max(n, 1), np.int32), sortnnz=np.zeros(max(n, 1), np.int32),
+                   tile_ptr=np.zeros(T + 1, np.int32), tile_len=np.zeros(max(T, 1), np.int32),
+                   tile_colidx=np.zeros(max(T, 1), np.int32), tile_format=np.zeros(max(T, 1), np.int32),
+                   sortrowidx=np.zeros(max(nz, 1), np.int32), sortrowindex=np.zeros(max(nz, 1), np.int32),
+                   sortval=np.zeros(max(nz, 1)), countrow=np.zeros(T + 1, np.int32),
+                   segmentoffset=np.zeros(S + 1, np.int32), lvl_col=np.zeros(max(L, 1), np.int32),
+                   lvl_base=np.zeros(max(L, 1), np.int32), lvl_tall=np.zeros(max(L, 1), np.int32))
+        order = ["perm", "sortnnz", "tile_ptr", "tile_len", "tile_colidx", "tile_format", "sortrowidx",
+                 "sortrowindex", "sortval", "countrow", "segmentoffset", "lvl_col", "lvl_base", "lvl_tall"]
+        check(lib.fl_matrix_export(self.handle, *[out[k].ctypes.data for k in order]), "fl_matrix_export")
+        sizes = dict(perm=n, sortnnz=n, tile_len=T, tile_colidx=T, tile_format=T, sortrowidx=nz, sortrowindex=nz,
+                     sortval=nz, lvl_col=L, lvl_base=L, lvl_tall=L)
+        for k, s in sizes.items():
+            out[k] = out[k][:s]
+        out.update(tilenum=T, segsum=S, L=L)
+        return out
+
+    def prepare(self, val_kind: int, stream=None):
+        check(lib.fl_matrix_prepare_values(self.handle, val_kind, stream.ptr if stream is not None else None),
+              "fl_matrix_prepare_values")
+
+    def permute_x(self, dx: DeviceArray, dsx: DeviceArray, stream=None):
+        fn = lib.fl_permute_x_f64 if dx.dtype == np.float64 else lib.fl_permute_x_f32
+        check(fn(self.handle, dx.ptr, dsx.ptr, stream.ptr if stream is not None else None), "fl_permute_x")
+
+    def spmv(self, dsx: DeviceArray, dy: DeviceArray, kernel=KERNEL_DROPIN, val_kind=VAL_F64, work=None, stream=None):
+        """y += A * sortx on the device."""
+        check(lib.fl_spmv(self.handle, kernel, val_kind, dsx.ptr, dy.ptr, work.ptr if work is not None else None,
+                          stream.ptr if stream is not None else None), "fl_spmv")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.fl_matrix_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------------- generators
+def gen_matrix(family: int, seed: int, m: int, n: int, params=()):
+    """Seeded synthetic CSC (include/fastload_gen.h) -> (m, n, colptr, rowidx, val)."""
+    pr = np.ascontiguousarray(params, np.float64) if len(params) else None
+    cp, ri, v = i32p(), i32p(), f64p()
+    nz = C.c_int()
+    check(lib.fl_gen_matrix(family, seed, m, n, _p(pr), 0 if pr is None else pr.size, C.byref(cp), C.byref(ri),
+                            C.byref(v), C.byref(nz)), "fl_gen_matrix")
+    if family == GEN_STENCIL27:
+        g = int(params[0]) if len(params) else 128
+        m = n = g ** 3
+    out = (m, n, _copy_out(cp, n + 1, np.int32), _copy_out(ri, nz.value, np.int32), _copy_out(v, nz.value, np.float64))
+    for p in (cp, ri, v):
+        lib.fl_free(C.cast(p, C.c_void_p))
+    return out
+
+
+def gen_vector(dist: int, seed: int, count: int) -> np.ndarray:
+    out = np.zeros(max(count, 1))
+    check(lib.fl_gen_vector(dist, seed, count, out.ctypes.data), "fl_gen_vector")
+    return out[:count]

@@ -1,0 +1,168 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the reference golden
+fixtures. Preprocessing must be bit-exact; y must match the FP64 oracle within 1e-12 relative
+(FP64 mode), 1e-5 (FP32 mode), 1e-4 (split hi+lo BF16) and 2^-7 (BF16 alone, error reported),
+relative to sum_j |a_rj x_j| per row; integer data must be exact in FP64."""
+import json
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_bind as ob
+from conftest import GOLDEN
+from gpu_util import device_y, export_matches_oracle, rel_err
+
+import fastload_amd as fl
+
+pytestmark = pytest.mark.gpu
+
+TOL = {fl.VAL_F64: 1e-12, fl.VAL_F32: 1e-5, fl.VAL_BF16X2: 1e-4, fl.VAL_BF16: 2.0 ** -7}
+
+
+def golden_1138():
+    return np.load(GOLDEN / "ref_layout_1138_bus.npz")
+
+
+# ------------------------------------------------------------ reference host API path
+def test_host_api_pipeline_1138_bus(gpu):
+    """mmio_allinone -> csr_to_csc -> ColSort -> formatTransform -> FastLoad_spmv, as src/main.cu."""
+    g = golden_1138()
+    m, n, nnz, sym, rp, ci, v = fl.mmio_allinone(str(GOLDEN / "1138_bus.mtx"))
+    assert (m, n, nnz) == (1138, 1138, 4054)
+    v = (np.arange(nnz) % 10).astype(np.float64)  # src/main.cu:68-69
+    cv, cr, cp, npc, _ = fl.csr_to_csc(m, n, v, rp, ci)
+    assert np.array_equal(cp, g["csc_colptr"]) and np.array_equal(cr, g["csc_rowidx"]) and np.array_equal(cv, g["csc_val"])
+    assert np.array_equal(npc, np.diff(g["csc_colptr"]))
+    x = (np.arange(n) % 10).astype(np.float64)  # src/main.cu:93-96
+    tr, tv, sn, sx, _ = fl.ColSort(m, n, npc, cp, cr, cv, x)
+    assert np.array_equal(sn, g["sortnnz"]) and np.array_equal(sx, g["sortx"])
+    assert np.array_equal(tr, g["tmp_row"]) and np.array_equal(tv, g["tmp_val"])
+    tiles, h_count, _, _ = fl.formatTransform(tr, tv, sn, nnz, m, n)
+    assert h_count == 14 and tiles.tilenum == 139
+    for k in ["tile_ptr", "tile_len", "tile_colidx", "tile_format", "sortrowidx", "sortval", "sortrowindex",
+              "countrow", "segmentoffset"]:
+        assert np.array_equal(getattr(tiles, k), g[f"w32_{k}"]), k
+    y_golden = ob.spmv_csr(m, rp, ci, v, x)
+    y, ms, passed = fl.FastLoad_spmv(tiles, nnz, n, m, sx, y_golden, "1138_bus")
+    assert passed and np.array_equal(y, y_golden) and ms > 0
+
+
+def test_host_api_csr2csc_random(gpu):
+    rng = np.random.default_rng(11)
+    for (m, n, d) in [(1, 1, 1.0), (37, 1000, 0.01), (2000, 3, 0.5), (5000, 4000, 0.002)]:
+        A = sp.random(m, n, density=d, random_state=rng, format="csr")
+        A.sort_indices()
+        cv, cr, cp, npc, _ = fl.csr_to_csc(m, n, A.data, A.indptr, A.indices)
+        rp2, ci2 = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+        e = ob.csr2csc(m, n, rp2, ci2, A.data)
+        assert np.array_equal(cp, e[0]) and np.array_equal(cr, e[1]) and np.array_equal(cv, e[2])
+
+
+def test_host_api_invalid_inputs(gpu):
+    cp = np.array([0, 1, 2], np.int32)
+    with pytest.raises(fl.FastLoadError):  # nnzPerCol disagrees with colptr
+        fl.ColSort(2, 2, np.array([2, 0], np.int32), cp, np.array([0, 1], np.int32), np.ones(2), np.ones(2))
+    with pytest.raises(fl.FastLoadError):  # unsorted lengths
+        fl.formatTransform(np.zeros(3, np.int32), np.zeros(3), np.array([2, 1], np.int32), 3, 2, 2)
+    with pytest.raises(fl.FastLoadError):  # row index out of range
+        fl.FastLoadMatrix(2, 2, cp, np.array([0, 5], np.int32), np.ones(2))
+
+
+# ------------------------------------------------------------------ device-resident path
+def _cases():
+    rng = np.random.default_rng(5)
+    cases = []
+    g = golden_1138()
+    cases.append(("1138_bus", int(g["m"]), int(g["n"]), g["csc_colptr"], g["csc_rowidx"], g["csc_val"], g["x"]))
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_SYMGEOM, 1138, 1138, 1138)
+    cases.append(("symgeom", m, n, cp, ri, v, fl.gen_vector(fl.VEC_UNIFORM, 1139, n)))
+    for name, fam, mm, nn, pr in [("zipf", fl.GEN_ZIPF, 3000, 5000, (1.8, 800)), ("banded", fl.GEN_BANDED, 700, 700, (37,)),
+                                  ("blockdiag", fl.GEN_BLOCKDIAG, 4096, 4096, (64, 0)),
+                                  ("uniform", fl.GEN_UNIFORM, 900, 20000, (3e-3,)),
+                                  ("zipfdense", fl.GEN_ZIPF_DENSE, 5000, 3000, (2.0, 100, 0.01, 2000))]:
+        m, n, cp, ri, v = fl.gen_matrix(fam, 7, mm, nn, pr)
+        cases.append((name, m, n, cp, ri, v, rng.standard_normal(n)))
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_STENCIL27, 9, 0, 0, (12,))
+    cases.append(("stencil12", m, n, cp, ri, v, rng.standard_normal(n)))
+    # edge cases: one column; empty columns; a column per row count; rectangular; all-empty
+    cases.append(("n1", 5, 1, np.array([0, 3], np.int32), np.array([0, 2, 4], np.int32), np.array([1., 2, 3]), np.ones(1)))
+    cases.append(("empty_cols", 4, 6, np.array([0, 0, 2, 2, 3, 3, 3], np.int32), np.array([1, 3, 0], np.int32),
+                  np.array([1., 2, 3]), np.arange(6.0)))
+    cases.append(("all_empty", 3, 4, np.zeros(5, np.int32), np.zeros(0, np.int32), np.zeros(0), np.ones(4)))
+    dense = np.arange(1.0, 1 + 70 * 130).reshape(70, 130) % 7  # dense-ish: full 32/64-tiles of one row
+    A = sp.csc_matrix(dense)
+    A.sort_indices()
+    cases.append(("dense70x130", 70, 130, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data, rng.standard_normal(130)))
+    return cases
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("W", [32, 64])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_device_layout_bit_exact(gpu, case, W):
+    name, m, n, cp, ri, v, x = case
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=W)
+    bad, t = export_matches_oracle(M, m, n, cp, ri, v, x, W)
+    assert not bad, f"{name} W={W}: arrays differ: {bad}"
+
+
+def test_device_layout_matches_reference_golden(gpu):
+    for fname in ("ref_layout_1138_bus.npz", "ref_layout_symgeom.npz"):
+        g = np.load(GOLDEN / fname)
+        for W in (32, 64):
+            M = fl.FastLoadMatrix(int(g["m"]), int(g["n"]), g["csc_colptr"], g["csc_rowidx"], g["csc_val"], width=W)
+            e = M.export()
+            keys = ["tile_ptr", "tile_len", "tile_colidx", "sortrowidx", "sortval", "sortrowindex", "countrow",
+                    "segmentoffset"] + (["tile_format"] if W == 32 else [])
+            for k in keys:
+                assert np.array_equal(e[k], g[f"w{W}_{k}"]), (fname, W, k)
+            assert np.array_equal(e["perm"], g["perm"])
+
+
+KERNELS = [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_DROPIN, fl.VAL_F64, 64), (fl.KERNEL_DROPIN, fl.VAL_F32, 32),
+           (fl.KERNEL_DROPIN, fl.VAL_F32, 64), (fl.KERNEL_FAST, fl.VAL_F64, 64), (fl.KERNEL_FAST, fl.VAL_F32, 64),
+           (fl.KERNEL_FAST, fl.VAL_BF16, 64), (fl.KERNEL_FAST, fl.VAL_BF16X2, 64)]
+KIDS = [f"{'dropin' if k == 0 else 'fast'}-{['f64', 'f32', 'bf16', 'bf16x2'][v]}-w{w}" for k, v, w in KERNELS]
+
+
+@pytest.mark.parametrize("kernel,kind,W", KERNELS, ids=KIDS)
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_spmv_parity(gpu, case, kernel, kind, W):
+    name, m, n, cp, ri, v, x = case
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=W)
+    y = device_y(M, x, kernel, kind)
+    y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
+    err = rel_err(y, y_or, scale)
+    assert err <= TOL[kind], f"{name}: rel err {err:.3e} > {TOL[kind]:.1e}"
+    if kind == fl.VAL_F64:
+        # FastLoad_CPU (same tiles, serial order) agrees to the same tolerance
+        t = ob.pipeline(m, n, cp, ri, v, x, W)
+        y_cpu = ob.fastload_cpu_f64(t, t["sortx"], m)
+        assert rel_err(y, y_cpu, scale) <= 1e-12
+
+
+@pytest.mark.parametrize("kernel,kind,W", [k for k in KERNELS if k[1] in (fl.VAL_F64, fl.VAL_F32)],
+                         ids=[i for i, k in zip(KIDS, KERNELS) if k[1] in (fl.VAL_F64, fl.VAL_F32)])
+def test_spmv_integer_data_exact(gpu, kernel, kind, W):
+    """The reference's exactness trick: values i%10, x i%10 -> every partial sum is an exact small
+    integer, so any summation order gives the same bits (src/main.cu:68-69,93-96, FastLoad_CPU.h:33-45)."""
+    g = golden_1138()
+    m, n = int(g["m"]), int(g["n"])
+    M = fl.FastLoadMatrix(m, n, g["csc_colptr"], g["csc_rowidx"], g["csc_val"], width=W)
+    y = device_y(M, g["x"], kernel, kind)
+    assert np.array_equal(y, g["w32_y"])
+
+
+def test_kat_9x9_gpu(gpu):
+    kat = json.loads((GOLDEN / "kat_9x9.json").read_text())
+    A = sp.csc_matrix(np.array(kat["A"], np.float64))
+    A.sort_indices()
+    for W in (32, 64):
+        M = fl.FastLoadMatrix(9, 9, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data, width=W)
+        for kernel, kind, w in KERNELS:
+            if w != W or kind in (fl.VAL_BF16,):
+                continue
+            y = device_y(M, np.ones(9), kernel, kind)
+            assert y.tolist() == kat["row_sums"], (W, kernel, kind)

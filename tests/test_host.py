@@ -1,0 +1,176 @@
+"""CPU-only checks of the product library: it loads, exports every symbol include/*.h declares,
+its host code (Matrix Market loader, generators) agrees with the oracle, and the index math of the
+FAST kernel's level walk reproduces the tile layout exactly (so it never reads out of bounds)."""
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import GOLDEN, ROOT
+
+import fastload_amd as fl
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for mt in re.finditer(r"\b(fl_[A-Za-z0-9_]+)\s*\(", text):
+            names.add(mt.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(ROOT / "fastload_amd" / "libfastload.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = declared_functions() - exported
+    assert not missing, missing
+    for name in declared_functions():
+        assert hasattr(fl.lib, name)
+
+
+def test_compat_header_compiles():
+    src = ROOT / "include" / "fastload_compat.hpp"
+    if not src.exists():
+        pytest.skip("no compat header")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-x", "c++", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_loader_matches_oracle_1138():
+    a = fl.mmio_allinone(str(GOLDEN / "1138_bus.mtx"))
+    b = ob.mmio_allinone(str(GOLDEN / "1138_bus.mtx"))
+    assert a[:4] == b[1:5]
+    for x, y in zip(a[4:], b[5:]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("text", [
+    "%%MatrixMarket matrix coordinate pattern general\n% c\n3 4 3\n1 4\n3 1\n2 2\n",
+    "%%MatrixMarket matrix coordinate integer symmetric\n3 3 3\n2 1 5\n3 3 -1\n3 1 7\n",
+    "%%MatrixMarket matrix coordinate complex hermitian\n2 2 2\n1 1 1.5 0\n2 1 2.5 -1\n",
+    "%%MatrixMarket matrix coordinate real skew-symmetric\n3 3 2\n2 1 1.5\n3 2 -2\n",
+    "%%MatrixMarket MATRIX Coordinate REAL General\n2 3 2\n1 3 1e-3\n2 1 -7.25\n",
+    "%%MatrixMarket matrix coordinate real general\n\n2 2 1\n2 2 3\n",
+])
+def test_loader_variants_match_oracle(tmp_path, text):
+    p = tmp_path / "a.mtx"
+    p.write_text(text)
+    a = fl.mmio_allinone(str(p))
+    b = ob.mmio_allinone(str(p))
+    assert b[0] == 0 and a[:4] == b[1:5]
+    for x, y in zip(a[4:], b[5:]):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("text,code", [
+    ("junk\n", -2), ("%%MatrixMarket matrix coordinate real general\n", -4),
+    ("%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1\n", -3),
+    ("%%MatrixMarket matrix coordinate real general\n2 2 1\n1 3 1\n", -5),
+    ("%%MatrixMarket matrix array real general\n1 1\n1\n", -6),
+])
+def test_loader_errors_match_oracle(tmp_path, text, code):
+    p = tmp_path / "b.mtx"
+    p.write_text(text)
+    with pytest.raises(fl.FastLoadError) as e:
+        fl.mmio_allinone(str(p))
+    assert e.value.rc == code == ob.mmio_allinone(str(p))[0]
+    with pytest.raises(fl.FastLoadError):
+        fl.mmio_allinone(str(p) + ".missing")
+
+
+def _rows_ascending(cp, ri):
+    for c in range(len(cp) - 1):
+        seg = ri[cp[c]:cp[c + 1]]
+        if len(seg) > 1 and not np.all(np.diff(seg) > 0):
+            return False
+    return True
+
+
+def test_generators_basic():
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_SYMGEOM, 1138, 1138, 1138)
+    import scipy.sparse as sp
+    A = sp.csc_matrix((v, ri, cp), shape=(m, n))
+    assert (A != A.T).nnz == 0  # structurally (and numerically) symmetric
+    assert np.all(A.diagonal() >= 3.0)
+    assert 3900 < len(ri) < 4300 and _rows_ascending(cp, ri)
+    offdiag = np.diff(cp) - 1
+    assert offdiag.max() <= 17 and 2.3 < offdiag.mean() < 2.9
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_STENCIL27, 5, 0, 0, (6,))
+    assert m == n == 216 and len(ri) == (6 + 2 * 5) ** 3 and _rows_ascending(cp, ri)
+    assert np.all((np.abs(v) >= 0.5) & (np.abs(v) <= 1.5))
+    for fam, params in [(fl.GEN_ZIPF, (1.8, 500)), (fl.GEN_BANDED, (9,)), (fl.GEN_BLOCKDIAG, (40, 7)),
+                        (fl.GEN_UNIFORM, (1e-2,)), (fl.GEN_ZIPF_DENSE, (2.0, 100, 0.01, 400))]:
+        m, n, cp, ri, v = fl.gen_matrix(fam, 3, 1000, 800, params)
+        assert cp[-1] == len(ri) and _rows_ascending(cp, ri) and ri.min() >= 0 and ri.max() < 1000
+        again = fl.gen_matrix(fam, 3, 1000, 800, params)
+        assert np.array_equal(again[3], ri) and np.array_equal(again[4], v)
+
+
+def test_vectors():
+    assert fl.gen_vector(fl.VEC_MOD10, 0, 25).tolist() == [i % 10 for i in range(25)]
+    u = fl.gen_vector(fl.VEC_UNIFORM, 5, 100000)
+    assert -1 <= u.min() and u.max() <= 1 and abs(u.mean()) < 0.01
+    g = fl.gen_vector(fl.VEC_NORMAL, 5, 100000)
+    assert abs(g.std() - 1) < 0.02
+
+
+# --------------------------------------------- FAST kernel level walk, emulated on the CPU
+def fast_walk(n, W, lvl_col, lvl_tall, tileoff, nnzoff, T, U=4):
+    """Mirror of the wave-uniform index math in k_spmv_fast64 (fastload_amd/csrc/fl_spmv.hip)."""
+    groups = (T + 63) // 64
+    group_level = [int(np.searchsorted(tileoff, g * 64, side="right") - 1) for g in range(groups)]
+    out = []
+    for gi in range(groups):
+        t0, t1 = gi * 64, min(gi * 64 + 64, T)
+        k = group_level[gi]
+        col, tall, noff = lvl_col[k], lvl_tall[k], nnzoff[k]
+        chunks = (n - col + W - 1) // W
+        local = t0 - tileoff[k]
+        j, e = local // tall, local % tall
+        for t in range(t0, t1, U):
+            for u in range(U):
+                if t + u < t1:
+                    width = n - col
+                    ln = width - (chunks - 1) * W if j == chunks - 1 else W
+                    out.append((noff + j * tall * W + e * ln, ln, col + W * j))
+                    e += 1
+                    if e == tall:
+                        e = 0
+                        j += 1
+                        if j == chunks and t + u + 1 < t1:
+                            k += 1
+                            col, tall, noff = lvl_col[k], lvl_tall[k], nnzoff[k]
+                            chunks = (n - col + W - 1) // W
+                            j = 0
+    return out
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_fast_walk_reproduces_layout(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 3000))
+    if seed % 3 == 0:
+        counts = rng.zipf(1.7, n).clip(0, 400)
+    elif seed % 3 == 1:
+        counts = rng.integers(0, 4, n)
+    else:
+        counts = np.full(n, 27)
+    s = np.sort(counts).astype(np.int32)
+    nnz = int(s.sum())
+    if nnz == 0:
+        return
+    t = ob.transform(n, nnz, s, np.zeros(nnz, np.int32), np.zeros(nnz), 64)
+    tiles_per_level = [(-(-(n - c) // 64)) * tl for c, tl in zip(t["lvl_col"], t["lvl_tall"])]
+    tileoff = np.concatenate([[0], np.cumsum(tiles_per_level)]).astype(np.int64)
+    nnzoff = np.concatenate([[0], np.cumsum([(n - c) * tl for c, tl in zip(t["lvl_col"], t["lvl_tall"])])])
+    walk = fast_walk(n, 64, t["lvl_col"], t["lvl_tall"], tileoff, nnzoff, t["tilenum"])
+    assert len(walk) == t["tilenum"]
+    ptr, ln, col = map(np.array, zip(*walk))
+    assert np.array_equal(ptr, t["tile_ptr"][:-1]) and np.array_equal(ln, t["tile_len"])
+    assert np.array_equal(col, t["tile_colidx"])
+    assert np.all(ptr + ln <= nnz) and np.all(col + ln <= n)
