@@ -1,13 +1,20 @@
 """FastLoad SpMV benchmark on one MI355X.
 
-Workload (default): the power-law configuration -- 4,194,304 x 4,194,304 CSC, column nnz Zipf
-(exponent 1.8423 so the mean is 16 with max 50,000; ~67M nnz), rows uniform without repeats,
-values and x N(0,1), seed 42. A step is one y = A*x (y zeroed, then one SpMV) with the matrix and
-x resident in HBM. Variants:
-  plain_f32 / plain_f64 : the reference per-tile algorithm on the reference 32-wide layout
-  fast_f32 / fast_bf16x2 / fast_bf16 : 64-wide tiles, level-table geometry, no per-tile metadata
-`value` = GB/s-equivalent (algorithmic FP32 bytes / step time) of the fastest variant whose
-error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported beside it.
+Headline workload (`value`): the power-law configuration -- 4,194,304 x 4,194,304 CSC, column
+nnz Zipf (exponent 1.8423 so the mean is 16 with max 50,000; ~67M nnz), rows uniform without
+repeats, values and x N(0,1), seed 42. A step is one y = A*x (y zeroed, then one SpMV) with the
+matrix and x resident in HBM.
+
+Variants (schedules x value storage):
+  plain_*  the reference per-tile algorithm on the reference 32-wide layout (drop-in)
+  fast_*   64-wide tiles, level-table geometry, one global atomic per segment
+  win_*    batches of 256 tiles, rows compacted to LDS slots, FP64 LDS sums, one atomic per row
+  pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
+  *_f32 / *_f64 / *_bf16 / *_bf16x2: value storage (split hi+lo BF16 recombined in FP64)
+`value` = GB/s-equivalent (algorithmic bytes with FP32 values / step time) of the fastest
+variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported beside it.
+The stencil configuration (128^3 27-point, 55.7M nnz) is timed too and reported under
+"stencil" with the error of every value storage.
 
 Algorithmic bytes per SpMV: B = nnz*(s_val + 4) + n*s_x + m*s_y.
 """
@@ -33,11 +40,27 @@ def b_alg(nnz, n, m, s_val, s_x, s_y):
     return nnz * (s_val + 4) + n * s_x + m * s_y
 
 
-def time_variant(fl, M, kernel, kind, dsx, dy, work, steps, warmup):
+VARIANTS = {
+    # name: (W, kernel, kind, s_val, batch_tiles, band_rows)
+    "plain_f32": (32, 0, 1, 4, 0, 0),
+    "plain_f64": (32, 0, 0, 8, 0, 0),
+    "fast_f32": (64, 1, 1, 4, 0, 0),
+    "win_f32": (64, 2, 1, 4, 256, 0),
+    "win_bf16x2": (64, 2, 3, 4, 256, 0),
+    "win_bf16": (64, 2, 2, 2, 256, 0),
+    "win_f64": (64, 2, 0, 8, 256, 0),
+    "pb_f32": (64, 3, 1, 4, 256, 16384),
+    "pb_bf16x2": (64, 3, 3, 4, 256, 16384),
+    "pb_bf16": (64, 3, 2, 2, 256, 16384),
+    "pb_f64": (64, 3, 0, 8, 256, 16384),
+}
+
+
+def time_variant(fl, M, kernel, kind, dsx, dy, steps, warmup):
     from fastload_amd import Event
     for _ in range(warmup):
         dy.zero()
-        M.spmv(dsx, dy, kernel, kind, work=work)
+        M.spmv(dsx, dy, kernel, kind)
     fl.synchronize()
     ev = [(Event(), Event()) for _ in range(steps)]
     t0, t1 = Event(), Event()
@@ -45,13 +68,70 @@ def time_variant(fl, M, kernel, kind, dsx, dy, work, steps, warmup):
     for a, b in ev:
         dy.zero()
         a.record()
-        M.spmv(dsx, dy, kernel, kind, work=work)
+        M.spmv(dsx, dy, kernel, kind)
         b.record()
     t1.record()
     fl.synchronize()
     step_ms = t0.elapsed_ms(t1) / steps
     kern = np.array([a.elapsed_ms(b) for a, b in ev])
     return step_ms, float(kern.mean()), float(np.median(kern))
+
+
+def run_workload(fl, name, m, n, cp, ri, v, x, variants, steps, warmup):
+    from fastload_amd import DeviceArray
+    nnz = int(cp[n])
+    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
+    mats = {}
+    for W in sorted({VARIANTS[k][0] for k in variants} | {32}):
+        t0 = time.time()
+        mats[W] = fl.FastLoadMatrix(m, n, dcp, dri, dv, width=W)
+        fl.synchronize()
+        log(f"[{name}] preprocessed W={W}: levels={mats[W].levels} tiles={mats[W].tilenum} in {time.time() - t0:.2f}s")
+    del dcp, dri, dv
+    dx64 = DeviceArray.from_host(x)
+    dx32 = DeviceArray.from_host(x.astype(np.float32))
+    sx = {}
+    for W, M in mats.items():
+        sx[(W, 8)] = DeviceArray(n, np.float64)
+        M.permute_x(dx64, sx[(W, 8)])
+        sx[(W, 4)] = DeviceArray(n, np.float32)
+        M.permute_x(dx32, sx[(W, 4)])
+    y64 = DeviceArray(m, np.float64)
+    y32 = DeviceArray(m, np.float32)
+    # FP64 reference result: the plain FP64 path (parity-tested against the FP64 oracle)
+    y64.zero()
+    mats[32].spmv(sx[(32, 8)], y64, fl.KERNEL_DROPIN, fl.VAL_F64)
+    y_ref = y64.to_host()
+    ref_norm = float(np.linalg.norm(y_ref))
+    res = {}
+    for vname in variants:
+        W, kernel, kind, s_val, bt, R = VARIANTS[vname]
+        M = mats[W]
+        if kind != fl.VAL_F64:
+            M.prepare(kind)
+        if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+            t0 = time.time()
+            M.prepare_schedule(kernel, bt, max(R, 1024))
+            fl.synchronize()
+            prep_s = time.time() - t0
+        else:
+            prep_s = 0.0
+        s_x = 8 if kind == fl.VAL_F64 else 4
+        ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
+        step_ms, kern_ms, kern_med = time_variant(fl, M, kernel, kind, sx[(W, s_x)], ydev, steps, warmup)
+        y = ydev.to_host().astype(np.float64)
+        err = float(np.linalg.norm(y - y_ref) / ref_norm) if ref_norm else 0.0
+        s_y = 8 if ydev is y64 else 4
+        B = b_alg(nnz, n, m, s_val, s_x, s_y)
+        B32 = b_alg(nnz, n, m, 4, 4, 4)
+        res[vname] = dict(step_ms=round(step_ms, 5), kernel_ms=round(kern_ms, 5), kernel_median_ms=round(kern_med, 5),
+                          gbs=round(B / step_ms / 1e6, 2), kernel_gbs=round(B / kern_ms / 1e6, 2),
+                          gbs_equiv_f32=round(B32 / step_ms / 1e6, 2), kernel_gbs_equiv_f32=round(B32 / kern_ms / 1e6, 2),
+                          bytes_alg=B, rel_err_l2=err, gflops=round(2 * nnz / step_ms / 1e6, 2),
+                          schedule_prep_s=round(prep_s, 3))
+        log(f"[{name}] {vname:11s} step {step_ms:.4f} ms kernel {kern_ms:.4f} ms  {res[vname]['gbs']:7.1f} GB/s "
+            f"(f32-equiv {res[vname]['gbs_equiv_f32']:7.1f})  err {err:.2e}")
+    return nnz, mats, res
 
 
 def cpu_baseline(M, m, n, nnz, budget_s=12.0):
@@ -84,80 +164,30 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,fast_bf16x2,fast_bf16")
+    ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_bf16x2,pb_bf16,pb_f64")
+    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_bf16x2,win_bf16,win_f64,pb_f32")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this FastLoad build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     import fastload_amd as fl
-    from fastload_amd import DeviceArray
 
     n = m = args.n
     t0 = time.time()
     _, _, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, args.seed, m, n, (1.8423, 50000))
     x = fl.gen_vector(fl.VEC_NORMAL, args.seed + 1, n)
-    nnz = int(cp[n])
-    log(f"generated {m}x{n} nnz={nnz} in {time.time() - t0:.1f}s")
+    log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
+    nnz, mats, res = run_workload(fl, "powerlaw", m, n, cp, ri, v, x, args.variants.split(","), args.steps, args.warmup)
+    del cp, ri, v
 
-    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
-    mats = {}
-    for W in (32, 64):
-        t0 = time.time()
-        mats[W] = fl.FastLoadMatrix(m, n, dcp, dri, dv, width=W)
-        fl.synchronize()
-        log(f"preprocessed W={W}: levels={mats[W].levels} tiles={mats[W].tilenum} in {time.time() - t0:.2f}s")
-    del dcp, dri, dv
-
-    dx64 = DeviceArray.from_host(x)
-    dx32 = DeviceArray.from_host(x.astype(np.float32))
-    sx = {}
-    for W, M in mats.items():
-        sx[(W, 8)] = DeviceArray(n, np.float64)
-        M.permute_x(dx64, sx[(W, 8)])
-        sx[(W, 4)] = DeviceArray(n, np.float32)
-        M.permute_x(dx32, sx[(W, 4)])
-    y64 = DeviceArray(m, np.float64)
-    y32 = DeviceArray(m, np.float32)
-    work = DeviceArray(2 * m, np.float32)
-
-    # FP64 reference result from the plain FP64 path (parity-tested against the FP64 oracle)
-    y64.zero()
-    mats[32].spmv(sx[(32, 8)], y64, fl.KERNEL_DROPIN, fl.VAL_F64)
-    y_ref = y64.to_host()
-    ref_norm = float(np.linalg.norm(y_ref))
-
-    table = {
-        "plain_f32": (32, fl.KERNEL_DROPIN, fl.VAL_F32, 4, 4, y32),
-        "plain_f64": (32, fl.KERNEL_DROPIN, fl.VAL_F64, 8, 8, y64),
-        "fast_f32": (64, fl.KERNEL_FAST, fl.VAL_F32, 4, 4, y32),
-        "fast_bf16x2": (64, fl.KERNEL_FAST, fl.VAL_BF16X2, 4, 4, y64),
-        "fast_bf16": (64, fl.KERNEL_FAST, fl.VAL_BF16, 2, 4, y32),
-    }
-    results = {}
-    for name in args.variants.split(","):
-        W, kernel, kind, s_val, s_x, ydev = table[name]
-        M = mats[W]
-        if kind != fl.VAL_F64:
-            M.prepare(kind)
-        step_ms, kern_ms, kern_med = time_variant(fl, M, kernel, kind, sx[(W, s_x if s_x == 8 else 4)], ydev,
-                                                  work, args.steps, args.warmup)
-        y = ydev.to_host().astype(np.float64)
-        err = float(np.linalg.norm(y - y_ref) / ref_norm) if ref_norm else 0.0
-        s_y = 8 if ydev is y64 else 4
-        B = b_alg(nnz, n, m, s_val, s_x, s_y)
-        B32 = b_alg(nnz, n, m, 4, 4, 4)
-        results[name] = dict(step_ms=round(step_ms, 5), kernel_ms=round(kern_ms, 5), kernel_median_ms=round(kern_med, 5),
-                             gbs=round(B / step_ms / 1e6, 2), kernel_gbs=round(B / kern_ms / 1e6, 2),
-                             gbs_equiv_f32=round(B32 / step_ms / 1e6, 2), bytes_alg=B, rel_err_l2=err,
-                             gflops=round(2 * nnz / step_ms / 1e6, 2))
-        log(f"{name:12s} step {step_ms:.4f} ms  kernel {kern_ms:.4f} ms  {results[name]['gbs']:.1f} GB/s "
-            f"(kernel {results[name]['kernel_gbs']:.1f})  err {err:.2e}")
-
-    ok = {k: r for k, r in results.items() if r["rel_err_l2"] <= 1e-4 and k != "plain_f64"}
-    best = max(ok, key=lambda k: ok[k]["gbs_equiv_f32"]) if ok else next(iter(results))
-    r = results[best]
-    plain = results.get("plain_f32")
+    ok = {k: r for k, r in res.items() if r["rel_err_l2"] <= 1e-4 and not k.startswith("plain")}
+    best = max(ok, key=lambda k: ok[k]["gbs_equiv_f32"]) if ok else next(iter(res))
+    r = res[best]
+    plain = res.get("plain_f32")
+    kind_names = {"f32": "f32", "f64": "f64", "bf16": "bf16 values, f32 products, f64 accumulate",
+                  "bf16x2": "split hi+lo bf16 values, f32 products, f64 accumulate"}
     out = {
         "metric": "SpMV HBM GB/s (% of peak): learned+mixed-prec vs plain FastLoad at stated err",
         "value": r["gbs_equiv_f32"],
@@ -169,17 +199,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fast_bf16x2": "bf16x2 values, f32 accumulate, f64 recombine", "fast_bf16": "bf16 values, f32 accumulate",
-                  "fast_f32": "f32", "plain_f32": "f32"}.get(best, "f32"),
+        "dtype": kind_names[best.split("_", 1)[1]],
         "data": "synthetic",
         "config": {"workload": f"power-law CSC SpMV {m}x{n}, Zipf(1.8423, max 50000) column nnz, seed {args.seed}",
                    "nnz": nnz, "variant": best, "rel_err_l2_vs_fp64": r["rel_err_l2"],
                    "plain_f32_gbs": plain["gbs"] if plain else None,
                    "speedup_vs_plain_f32": round(plain["step_ms"] / r["step_ms"], 3) if plain else None},
-        "roofline": {"bound": "hbm", "achieved": r["kernel_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["kernel_gbs"] / PEAK_GBS, 4), "traffic": None},
-        "variants": results,
+        "roofline": {"bound": "hbm", "achieved": r["kernel_gbs_equiv_f32"], "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4), "traffic": None},
+        "variants": res,
     }
+    if not args.no_stencil:
+        _, _, scp, sri, sv = fl.gen_matrix(fl.GEN_STENCIL27, 128, 0, 0, (128,))
+        sn = sm = 128 ** 3
+        sx = fl.gen_vector(fl.VEC_NORMAL, 129, sn)
+        snnz, smats, sres = run_workload(fl, "stencil", sm, sn, scp, sri, sv, sx, args.stencil_variants.split(","),
+                                         args.steps, args.warmup)
+        out["stencil"] = {"workload": "27-point stencil on 128^3, |v| U[0.5,1.5] random sign, seed 128", "nnz": snnz,
+                          "variants": sres}
+        del smats
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mats[32], m, n, nnz)
     print(json.dumps(out), flush=True)

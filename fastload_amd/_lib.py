@@ -60,6 +60,7 @@ SIGNATURES = {
     "fl_permute_x_f32": (C.c_int, [vp, vp, vp, vp]),
     "fl_matrix_prepare_values": (C.c_int, [vp, C.c_int, vp]),
     "fl_spmv": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
+    "fl_matrix_prepare_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "fl_gen_matrix": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int, vp, C.c_int, C.POINTER(i32p), C.POINTER(i32p),
                                 C.POINTER(f64p), i32p]),
     "fl_gen_vector": (C.c_int, [C.c_int, C.c_uint64, C.c_int, vp]),
@@ -69,7 +70,7 @@ SIGNATURES = {
 def _load():
     if not LIB_PATH.exists():
         raise ImportError(f"fastload_amd: native library {LIB_PATH} is missing; build it with "
-                          f"`python -m fastload_amd.build` (hipcc, gfx950)")
+                          f"`python fastload_amd/build.py` (hipcc, gfx950)")
     lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -16,7 +16,7 @@ from ._lib import FastLoadError, MatrixInfo, SlideMatrix, check, i32p, f64p, lib
 from .device import DeviceArray
 
 VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2 = 0, 1, 2, 3
-KERNEL_DROPIN, KERNEL_FAST = 0, 1
+KERNEL_DROPIN, KERNEL_FAST, KERNEL_WIN, KERNEL_PB = 0, 1, 2, 3
 
 GEN_SYMGEOM, GEN_ZIPF, GEN_STENCIL27, GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF_DENSE = range(7)
 VEC_NORMAL, VEC_UNIFORM, VEC_MOD10 = range(3)
@@ -186,6 +186,10 @@ class FastLoadMatrix:
     def prepare(self, val_kind: int, stream=None):
         check(lib.fl_matrix_prepare_values(self.handle, val_kind, stream.ptr if stream is not None else None),
               "fl_matrix_prepare_values")
+
+    def prepare_schedule(self, kernel: int, batch_tiles: int = 256, band_rows: int = 16384, stream=None):
+        check(lib.fl_matrix_prepare_schedule(self.handle, kernel, batch_tiles, band_rows,
+                                             stream.ptr if stream is not None else None), "fl_matrix_prepare_schedule")
 
     def permute_x(self, dx: DeviceArray, dsx: DeviceArray, stream=None):
         fn = lib.fl_permute_x_f64 if dx.dtype == np.float64 else lib.fl_permute_x_f32

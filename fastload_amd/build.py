@@ -1,5 +1,6 @@
 """Builds fastload_amd/libfastload.so in-tree: HIP sources with hipcc for gfx950, host-only
-sources with g++, linked by hipcc. Run `python -m fastload_amd.build` (or __graft_entry__.build()).
+sources with g++, linked by hipcc. Run `python fastload_amd/build.py` (or __graft_entry__.build());
+the script does not import the package, so a stale library never blocks a rebuild.
 """
 from __future__ import annotations
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "fl_common.h"
+#include "fl_walk.h"
 
 struct fl_matrix {
     int m = 0, n = 0, nnz = 0, W = 32;
@@ -25,9 +26,39 @@ struct fl_matrix {
     fl::DevBuf<uint16_t> bf16_hi, bf16_lo;
     // FAST kernel: level index of the first tile of every 64-tile group
     fl::DevBuf<int> group_level;
+
+    // Batched schedules (W = 64): a batch is `bt` consecutive tiles (<= bt*64 entries).
+    int bt = 0, nbatches = 0;
+    fl::DevBuf<int> batch_of;  // [nnz] batch index of every entry (tile order), built on demand
+    // WIN: rows of each batch compacted to LDS slots.
+    //   win_slot[g]  slot of entry g inside its batch; win_rows[win_off[b] + s] = row of slot s
+    bool win_ready = false;
+    fl::DevBuf<uint16_t> win_slot;
+    fl::DevBuf<int> win_rows, win_off;
+    // PB (two-phase propagation blocking): rows split in bands of pb_R rows.
+    //   phase 1 stages products in LDS at pb_lslot[g] (band-major inside the batch) and copies
+    //   runs (batch, band) to the band-major product array; phase 2 accumulates each band.
+    int pb_R = 0, nbands = 0, nruns = 0;
+    bool pb_ready = false;
+    fl::DevBuf<uint16_t> pb_lslot, pb_rowlocal;
+    fl::DevBuf<int> pb_band_start, pb_run_off, pb_run_lstart, pb_run_dest, pb_run_len;
+    fl::DevBuf<float> pb_bins32;
+    fl::DevBuf<double> pb_bins64;
 };
 
 namespace fl {
+
+inline LevelTab level_tab(const fl_matrix *A) {
+    LevelTab lt;
+    lt.col = A->lvl_col.p;
+    lt.tall = A->lvl_tall.p;
+    lt.tileoff = A->lvl_tileoff.p;
+    lt.nnzoff = A->lvl_nnzoff.p;
+    lt.group_level = A->group_level.p;
+    lt.n = A->n;
+    lt.L = A->L;
+    return lt;
+}
 
 // Pipeline stages shared by the host API and the device-resident builder.
 // Stable column sort: perm/sortnnz from per-column counts (device arrays of n ints).

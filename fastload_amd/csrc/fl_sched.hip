@@ -1,0 +1,515 @@
+// Batched FastLoad schedules for the 64-wide layout: y partials accumulate in LDS in FP64
+// (ds_add_f64 is several times faster than the FP32 LDS atomic on gfx950), so scattered
+// global atomics -- one per segment in the reference kernel (src/FastLoad_GPU.h:186,224,267)
+// -- are replaced by LDS work.
+//
+// A batch is `bt` consecutive tiles (<= bt*64 entries), handled by one 1024-thread workgroup;
+// each of its 16 waves walks bt/16 consecutive tiles with the level table (fl_walk.h), so x
+// is read exactly as in FastLoad: 64 consecutive sorted columns per tile.
+//
+// WIN (rows with locality: stencils, bands, blocks): the distinct rows of a batch are
+//   compacted to LDS slots at preprocessing time; entries store a 2-byte slot instead of a
+//   4-byte row; products are summed per slot with ds_add_f64 and each slot is flushed with
+//   one global atomic (consecutive rows -> coalesced atomics).
+// PB (rows without locality, e.g. uniform random rows): two-phase propagation blocking.
+//   Phase 1 stages the batch's products in LDS in band-major order (bands of R rows) and
+//   copies each (batch, band) run contiguously into a band-major product array. Phase 2 gives
+//   each band one workgroup that sums its products per row in LDS (ds_add_f64) and updates y
+//   once per row with a coalesced read-modify-write -- no global atomics at all.
+#include <algorithm>
+
+#include "fl_matrix.h"
+
+namespace fl {
+
+void exclusive_scan_i32(const int *in, int *out, int count, hipStream_t s);
+
+static unsigned sg(long long count) { return std::max(1u, std::min<unsigned>(grid_for(count), 65536u)); }
+
+#define GRID_STRIDE(i, count) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (count); i += (long long)gridDim.x * blockDim.x)
+
+// ------------------------------------------------------------------------ preprocessing
+__global__ void k_batch_of(int T, int bt, const int *__restrict__ tile_ptr, const int *__restrict__ tile_len,
+                           int *__restrict__ batch_of) {
+    GRID_STRIDE(g, (long long)T * 64) {
+        const long long t = g >> 6;
+        const int l = (int)(g & 63);
+        if (l < tile_len[t]) batch_of[tile_ptr[t] + l] = (int)(t / bt);
+    }
+}
+
+__global__ void k_iota2(int *__restrict__ v, int count) {
+    GRID_STRIDE(i, count) v[i] = (int)i;
+}
+
+__global__ void k_key_gather(const int *__restrict__ src, const int *__restrict__ order, uint32_t *__restrict__ key,
+                             int count) {
+    GRID_STRIDE(i, count) key[i] = (uint32_t)src[order[i]];
+}
+
+__global__ void k_key_copy(const int *__restrict__ src, uint32_t *__restrict__ key, int count, int div) {
+    GRID_STRIDE(i, count) key[i] = (uint32_t)(src[i] / div);
+}
+
+__global__ void k_key_batch_band(const int *__restrict__ batch_of, const int *__restrict__ row, int R, int nbands,
+                                 uint32_t *__restrict__ key, int count) {
+    GRID_STRIDE(i, count) key[i] = (uint32_t)batch_of[i] * (uint32_t)nbands + (uint32_t)(row[i] / R);
+}
+
+// WIN: heads of (batch, row) runs in (batch, row, g) order
+__global__ void k_win_heads(const uint32_t *__restrict__ kb, const int *__restrict__ order, const int *__restrict__ row,
+                            int *__restrict__ head, int count) {
+    GRID_STRIDE(i, count) {
+        int h = 1;
+        if (i > 0) h = (kb[i] != kb[i - 1]) || (row[order[i]] != row[order[i - 1]]);
+        head[i] = h;
+    }
+}
+
+__global__ void k_win_off(const uint32_t *__restrict__ kb, const int *__restrict__ d, int *__restrict__ win_off,
+                          int count) {
+    GRID_STRIDE(i, count) if (i == 0 || kb[i] != kb[i - 1]) win_off[kb[i]] = d[i];
+}
+
+__global__ void k_win_slot(const uint32_t *__restrict__ kb, const int *__restrict__ order, const int *__restrict__ row,
+                           const int *__restrict__ head, const int *__restrict__ d, const int *__restrict__ win_off,
+                           uint16_t *__restrict__ slot, int *__restrict__ win_rows, int count) {
+    GRID_STRIDE(i, count) {
+        const int g = order[i];
+        const int di = d[i] + head[i] - 1;  // index of this entry's (batch, row) run
+        slot[g] = (uint16_t)(di - win_off[kb[i]]);
+        if (head[i]) win_rows[d[i]] = row[g];
+    }
+}
+
+// PB: dest[g] = band-major position; rowlocal[pos] = row - band*R
+__global__ void k_pb_dest(const int *__restrict__ orderA, const int *__restrict__ row, int R, int *__restrict__ dest,
+                          uint16_t *__restrict__ rowlocal, int count) {
+    GRID_STRIDE(i, count) {
+        const int g = orderA[i];
+        dest[g] = (int)i;
+        rowlocal[i] = (uint16_t)(row[g] % R);
+    }
+}
+
+__global__ void k_count_keys(const uint32_t *__restrict__ k, int count, int *__restrict__ counts) {
+    GRID_STRIDE(i, count) atomicAdd(&counts[k[i]], 1);
+}
+
+__global__ void k_pb_heads(const uint32_t *__restrict__ kb, int *__restrict__ head, int count) {
+    GRID_STRIDE(i, count) head[i] = (i == 0) || (kb[i] != kb[i - 1]);
+}
+
+__global__ void k_pb_runs(const uint32_t *__restrict__ kb, const int *__restrict__ orderB, const int *__restrict__ head,
+                          const int *__restrict__ runidx, const int *__restrict__ dest, const int *__restrict__ tile_ptr,
+                          int bt, int nbands, uint16_t *__restrict__ lslot, int *__restrict__ run_lstart,
+                          int *__restrict__ run_dest, int *__restrict__ run_pos, int *__restrict__ run_off,
+                          int count) {
+    GRID_STRIDE(i, count) {
+        const int batch = (int)(kb[i] / (uint32_t)nbands);
+        const int bstart = tile_ptr[batch * bt];
+        const int g = orderB[i];
+        lslot[g] = (uint16_t)(i - bstart);
+        if (head[i]) {
+            const int r = runidx[i];
+            run_lstart[r] = (int)i - bstart;
+            run_dest[r] = dest[g];
+            run_pos[r] = (int)i;
+            if (i == 0 || kb[i] / (uint32_t)nbands != kb[i - 1] / (uint32_t)nbands) run_off[batch] = r;
+        }
+    }
+}
+
+__global__ void k_run_len(const int *__restrict__ run_pos, int nruns, int total, int *__restrict__ run_len) {
+    GRID_STRIDE(r, nruns) run_len[r] = ((r + 1 < nruns) ? run_pos[r + 1] : total) - run_pos[r];
+}
+
+// Validators: every index the kernels will use must be inside the buffers they assume.
+__global__ void k_check_win(const int *__restrict__ batch_of, const uint16_t *__restrict__ slot,
+                            const int *__restrict__ win_off, int cap, int nnz, int m,
+                            const int *__restrict__ win_rows, int *__restrict__ flag) {
+    GRID_STRIDE(g, nnz) {
+        const int b = batch_of[g];
+        const int ns = win_off[b + 1] - win_off[b];
+        if (ns <= 0 || ns > cap || (int)slot[g] >= ns) {
+            atomicOr(flag, 1);
+            continue;
+        }
+        const int r = win_rows[win_off[b] + slot[g]];
+        if (r < 0 || r >= m) atomicOr(flag, 2);
+    }
+}
+
+__global__ void k_check_pb(const int *__restrict__ batch_of, const uint16_t *__restrict__ lslot,
+                           const int *__restrict__ tile_ptr, int bt, int T, int nnz,
+                           const int *__restrict__ run_off, const int *__restrict__ run_lstart,
+                           const int *__restrict__ run_dest, const int *__restrict__ run_len, int nb,
+                           int *__restrict__ flag) {
+    GRID_STRIDE(g, nnz) {
+        const int b = batch_of[g];
+        const int bsize = tile_ptr[min((b + 1) * bt, T)] - tile_ptr[b * bt];
+        if ((int)lslot[g] >= bsize) atomicOr(flag, 4);
+    }
+    GRID_STRIDE(b, nb) {
+        const int bsize = tile_ptr[min((int)(b + 1) * bt, T)] - tile_ptr[b * bt];
+        for (int r = run_off[b]; r < run_off[b + 1]; ++r)
+            if (run_len[r] <= 0 || run_lstart[r] < 0 || run_lstart[r] + run_len[r] > bsize || run_dest[r] < 0 ||
+                run_dest[r] + run_len[r] > nnz)
+                atomicOr(flag, 8);
+    }
+}
+
+static int read1(const int *d, hipStream_t s) {
+    int h = 0;
+    FL_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+static void write1(int *d, int v, hipStream_t s) {
+    FL_HIP(hipMemcpyAsync(d, &v, sizeof(int), hipMemcpyHostToDevice, s));
+    FL_HIP(hipStreamSynchronize(s));
+}
+
+static void ensure_batches(fl_matrix *A, int bt, hipStream_t s) {
+    if (A->bt == bt && A->batch_of.p) return;
+    A->bt = bt;
+    A->nbatches = (A->tilenum + bt - 1) / bt;
+    A->win_ready = A->pb_ready = false;
+    A->batch_of.alloc(std::max(A->nnz, 1));
+    if (A->tilenum > 0) {
+        k_batch_of<<<sg((long long)A->tilenum * 64), kBlock, 0, s>>>(A->tilenum, bt, A->tile_ptr.p, A->tile_len.p,
+                                                                     A->batch_of.p);
+        FL_LAUNCH_CHECK();
+    }
+}
+
+void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
+    ensure_batches(A, bt, s);
+    if (A->win_ready) return;
+    const int nnz = A->nnz, nb = A->nbatches;
+    A->win_slot.alloc(std::max(nnz, 1));
+    A->win_off.alloc((size_t)nb + 1);
+    if (nnz == 0) {
+        A->win_rows.alloc(1);
+        FL_HIP(hipMemsetAsync(A->win_off.p, 0, sizeof(int) * ((size_t)nb + 1), s));
+        A->win_ready = true;
+        return;
+    }
+    DevBuf<uint32_t> key(nnz);
+    DevBuf<int> order(nnz), head(nnz), d((size_t)nnz + 1);
+    // (row, g) order, then stable by batch -> (batch, row, g)
+    k_key_copy<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, key.p, nnz, 1);
+    k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(A->m - 1, 0)), s);
+    k_key_gather<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, order.p, key.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(nb - 1, 0)), s);
+    k_win_heads<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, A->sortrowidx.p, head.p, nnz);
+    FL_LAUNCH_CHECK();
+    exclusive_scan_i32(head.p, d.p, nnz, s);
+    const int D = read1(d.p + nnz, s);
+    A->win_rows.alloc(std::max(D, 1));
+    k_win_off<<<sg(nnz), kBlock, 0, s>>>(key.p, d.p, A->win_off.p, nnz);
+    FL_LAUNCH_CHECK();
+    write1(A->win_off.p + nb, D, s);
+    k_win_slot<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, A->sortrowidx.p, head.p, d.p, A->win_off.p, A->win_slot.p,
+                                          A->win_rows.p, nnz);
+    FL_LAUNCH_CHECK();
+    DevBuf<int> flag(1);
+    FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    k_check_win<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->win_slot.p, A->win_off.p, bt * 64, nnz, A->m,
+                                           A->win_rows.p, flag.p);
+    FL_LAUNCH_CHECK();
+    if (read1(flag.p, s)) {
+        set_error("WIN schedule: preprocessing produced out-of-range slots");
+        throw HipError{hipErrorInvalidValue};
+    }
+    A->win_ready = true;
+}
+
+void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
+    ensure_batches(A, bt, s);
+    if (A->pb_ready && A->pb_R == R) return;
+    const int nnz = A->nnz, nb = A->nbatches, m = A->m;
+    const int nbands = std::max((m + R - 1) / R, 1);
+    if ((long long)nb * nbands > 0x7fffffffLL) {
+        set_error("PB schedule: %d batches x %d bands exceed 31-bit keys", nb, nbands);
+        throw HipError{hipErrorInvalidValue};
+    }
+    A->pb_R = R;
+    A->nbands = nbands;
+    A->pb_lslot.alloc(std::max(nnz, 1));
+    A->pb_rowlocal.alloc(std::max(nnz, 1));
+    A->pb_band_start.alloc((size_t)nbands + 1);
+    A->pb_run_off.alloc((size_t)nb + 1);
+    if (nnz == 0) {
+        FL_HIP(hipMemsetAsync(A->pb_band_start.p, 0, sizeof(int) * ((size_t)nbands + 1), s));
+        FL_HIP(hipMemsetAsync(A->pb_run_off.p, 0, sizeof(int) * ((size_t)nb + 1), s));
+        A->nruns = 0;
+        A->pb_ready = true;
+        return;
+    }
+    DevBuf<uint32_t> key(nnz);
+    DevBuf<int> order(nnz), dest(nnz), head(nnz), runidx((size_t)nnz + 1), counts(nbands);
+    // A: band-major order (band, g)
+    k_key_copy<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, key.p, nnz, R);
+    k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)(nbands - 1)), s);
+    k_pb_dest<<<sg(nnz), kBlock, 0, s>>>(order.p, A->sortrowidx.p, R, dest.p, A->pb_rowlocal.p, nnz);
+    FL_HIP(hipMemsetAsync(counts.p, 0, sizeof(int) * nbands, s));
+    k_count_keys<<<sg(nnz), kBlock, 0, s>>>(key.p, nnz, counts.p);
+    FL_LAUNCH_CHECK();
+    exclusive_scan_i32(counts.p, A->pb_band_start.p, nbands, s);
+    // B: (batch, band, g) order -> LDS staging slots and runs
+    k_key_batch_band<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->sortrowidx.p, R, nbands, key.p, nnz);
+    k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)(nb * nbands - 1)), s);
+    k_pb_heads<<<sg(nnz), kBlock, 0, s>>>(key.p, head.p, nnz);
+    FL_LAUNCH_CHECK();
+    exclusive_scan_i32(head.p, runidx.p, nnz, s);
+    const int nruns = read1(runidx.p + nnz, s);
+    A->nruns = nruns;
+    A->pb_run_lstart.alloc(nruns);
+    A->pb_run_dest.alloc(nruns);
+    A->pb_run_len.alloc(nruns);
+    DevBuf<int> run_pos(nruns);
+    k_pb_runs<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, head.p, runidx.p, dest.p, A->tile_ptr.p, A->bt, nbands,
+                                         A->pb_lslot.p, A->pb_run_lstart.p, A->pb_run_dest.p, run_pos.p,
+                                         A->pb_run_off.p, nnz);
+    k_run_len<<<sg(nruns), kBlock, 0, s>>>(run_pos.p, nruns, nnz, A->pb_run_len.p);
+    FL_LAUNCH_CHECK();
+    write1(A->pb_run_off.p + nb, nruns, s);
+    DevBuf<int> flag(1);
+    FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->tile_ptr.p, A->bt, A->tilenum, nnz,
+                                          A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, nb,
+                                          flag.p);
+    FL_LAUNCH_CHECK();
+    const int total = read1(A->pb_band_start.p + nbands, s);
+    if (read1(flag.p, s) || total != nnz) {
+        set_error("PB schedule: preprocessing produced out-of-range runs or slots");
+        throw HipError{hipErrorInvalidValue};
+    }
+    A->pb_ready = true;
+}
+
+// ------------------------------------------------------------------------------ kernels
+constexpr int kBigBlock = 1024;
+
+template <typename V>
+struct Vload;
+template <>
+struct Vload<double> {
+    static __device__ __forceinline__ double get(const double *p, int i) { return p[i]; }
+};
+template <>
+struct Vload<float> {
+    static __device__ __forceinline__ float get(const float *p, int i) { return p[i]; }
+};
+template <>
+struct Vload<uint16_t> {
+    static __device__ __forceinline__ float get(const uint16_t *p, int i) { return __uint_as_float((uint32_t)p[i] << 16); }
+};
+
+// Product of entry g with x[cx + lane], in FP64 (FP64 mode) or as the FP64 sum of FP32
+// products (FP32 / BF16 / split hi+lo BF16 storage).
+template <typename V, typename X, bool SPLIT>
+__device__ __forceinline__ double product(const V *__restrict__ val, const V *__restrict__ val_lo,
+                                          const X *__restrict__ x, int g, int xi) {
+    if constexpr (sizeof(X) == 8) {
+        return (double)Vload<V>::get(val, g) * x[xi];
+    } else {
+        const float xv = x[xi];
+        const float p = Vload<V>::get(val, g) * xv;
+        if constexpr (SPLIT) return (double)p + (double)(Vload<V>::get(val_lo, g) * xv);
+        return (double)p;
+    }
+}
+
+__device__ __forceinline__ void gatomic(float *p, double v) { unsafeAtomicAdd(p, (float)v); }
+__device__ __forceinline__ void gatomic(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// Per-wave tile loop over this batch: U tiles in flight, fn(g, xi) for every active lane.
+template <int U, typename F>
+__device__ __forceinline__ void batch_tiles(const LevelTab &lt, int T_, int bt, int batch, F &&fn) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tpw = bt / (kBigBlock / 64);
+    const int t0 = batch * bt + wave * tpw;
+    const int t1 = min(t0 + tpw, T_);
+    if (t0 >= t1) return;
+    LevelWalk s;
+    s.seek(lt, t0);
+    for (int t = t0; t < t1; t += U) {
+        int ptr[U], cx[U], len[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t + u < t1) s.next(lt, ptr[u], cx[u], len[u], t + u + 1 < t1);
+            else { ptr[u] = 0; cx[u] = 0; len[u] = 0; }
+        }
+        fn(ptr, cx, len, lane);
+    }
+}
+
+template <typename V, typename X, typename Y, bool SPLIT, int U>
+__global__ __launch_bounds__(kBigBlock) void k_win(int T_, int bt, LevelTab lt, const uint16_t *__restrict__ slot,
+                                                   const V *__restrict__ val, const V *__restrict__ val_lo,
+                                                   const X *__restrict__ x, const int *__restrict__ win_rows,
+                                                   const int *__restrict__ win_off, Y *__restrict__ y) {
+    extern __shared__ double acc[];
+    const int b = blockIdx.x;
+    const int s0 = win_off[b], ns = win_off[b + 1] - s0;
+    for (int i = threadIdx.x; i < ns; i += kBigBlock) acc[i] = 0.0;
+    __syncthreads();
+    batch_tiles<U>(lt, T_, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+        int sl[U];
+        double p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool act = lane < len[u];
+            const int g = ptr[u] + lane;
+            sl[u] = act ? (int)slot[g] : 0;
+            p[u] = act ? product<V, X, SPLIT>(val, val_lo, x, g, cx[u] + lane) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (lane < len[u]) atomicAdd(&acc[sl[u]], p[u]);
+    });
+    __syncthreads();
+    for (int i = threadIdx.x; i < ns; i += kBigBlock) gatomic(&y[win_rows[s0 + i]], acc[i]);
+}
+
+template <typename V, typename X, typename P, bool SPLIT, int U>
+__global__ __launch_bounds__(kBigBlock) void k_pb1(int T_, int bt, LevelTab lt, const uint16_t *__restrict__ lslot,
+                                                   const V *__restrict__ val, const V *__restrict__ val_lo,
+                                                   const X *__restrict__ x, const int *__restrict__ run_off,
+                                                   const int *__restrict__ run_lstart, const int *__restrict__ run_dest,
+                                                   const int *__restrict__ run_len, P *__restrict__ bins) {
+    extern __shared__ double stage_raw[];
+    P *stage = reinterpret_cast<P *>(stage_raw);
+    const int b = blockIdx.x;
+    batch_tiles<U>(lt, T_, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+        int sl[U];
+        double p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool act = lane < len[u];
+            const int g = ptr[u] + lane;
+            sl[u] = act ? (int)lslot[g] : 0;
+            p[u] = act ? product<V, X, SPLIT>(val, val_lo, x, g, cx[u] + lane) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (lane < len[u]) stage[sl[u]] = (P)p[u];
+    });
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = run_off[b], r1 = run_off[b + 1];
+    for (int r = r0 + wave; r < r1; r += kBigBlock / 64) {
+        const int len = run_len[r], ls = run_lstart[r], d = run_dest[r];
+        for (int i = lane; i < len; i += 64) bins[d + i] = stage[ls + i];
+    }
+}
+
+template <typename P, typename Y, int U>
+__global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
+                                                   const P *__restrict__ bins, const uint16_t *__restrict__ rowlocal,
+                                                   Y *__restrict__ y) {
+    extern __shared__ double acc[];
+    const int b = blockIdx.x;
+    for (int i = threadIdx.x; i < R; i += kBigBlock) acc[i] = 0.0;
+    __syncthreads();
+    const int i0 = band_start[b], i1 = band_start[b + 1];
+    int i = i0 + threadIdx.x;
+    for (; i + (U - 1) * kBigBlock < i1; i += U * kBigBlock) {
+        P v[U];
+        int rl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v[u] = bins[i + u * kBigBlock];
+            rl[u] = rowlocal[i + u * kBigBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) atomicAdd(&acc[rl[u]], (double)v[u]);
+    }
+    for (; i < i1; i += kBigBlock) atomicAdd(&acc[rowlocal[i]], (double)bins[i]);
+    __syncthreads();
+    const int r0 = b * R, rows = min(R, m - r0);
+    for (int r = threadIdx.x; r < rows; r += kBigBlock) y[r0 + r] += (Y)acc[r];
+}
+
+// ------------------------------------------------------------------------------ launch
+template <typename K>
+static void allow_lds(K kern, size_t bytes) {
+    FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+template <typename V, typename X, typename Y, bool SPLIT>
+static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x, Y *y, hipStream_t s) {
+    constexpr int U = 4;
+    const size_t lds = (size_t)A->bt * 64 * sizeof(double);
+    auto kern = k_win<V, X, Y, SPLIT, U>;
+    allow_lds(kern, lds);
+    kern<<<A->nbatches, kBigBlock, lds, s>>>(A->tilenum, A->bt, level_tab(A), A->win_slot.p, val, lo, x, A->win_rows.p,
+                                             A->win_off.p, y);
+    FL_LAUNCH_CHECK();
+}
+
+template <typename V, typename X, typename P, typename Y, bool SPLIT>
+static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x, P *bins, Y *y, hipStream_t s) {
+    constexpr int U = 4;
+    const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
+    auto k1 = k_pb1<V, X, P, SPLIT, U>;
+    allow_lds(k1, std::max<size_t>(lds1, 8));
+    k1<<<A->nbatches, kBigBlock, lds1, s>>>(A->tilenum, A->bt, level_tab(A), A->pb_lslot.p, val, lo, x,
+                                            A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p,
+                                            bins);
+    FL_LAUNCH_CHECK();
+    const size_t lds2 = (size_t)A->pb_R * sizeof(double);
+    auto k2 = k_pb2<P, Y, 8>;
+    allow_lds(k2, lds2);
+    k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p, bins, A->pb_rowlocal.p, y);
+    FL_LAUNCH_CHECK();
+}
+
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, hipStream_t s) {
+    const bool win = (kernel == FL_KERNEL_WIN);
+    if (win && !A->win_ready) { set_error("fl_spmv: WIN schedule not prepared"); return FL_ERR_INVALID; }
+    if (!win && !A->pb_ready) { set_error("fl_spmv: PB schedule not prepared"); return FL_ERR_INVALID; }
+    if (kind == FL_VAL_F32 && !A->val32.p) { set_error("fl_spmv: FP32 values not prepared"); return FL_ERR_INVALID; }
+    if ((kind == FL_VAL_BF16 || kind == FL_VAL_BF16X2) && !A->bf16_hi.p) {
+        set_error("fl_spmv: BF16 values not prepared");
+        return FL_ERR_INVALID;
+    }
+    if (!win) {
+        if (kind == FL_VAL_F64) { if (!A->pb_bins64.p) A->pb_bins64.alloc(std::max(A->nnz, 1)); }
+        else if (!A->pb_bins32.p) A->pb_bins32.alloc(std::max(A->nnz, 1));
+    }
+    switch (kind) {
+        case FL_VAL_F64:
+            if (win) launch_win<double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, (double *)y, s);
+            else launch_pb<double, double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, A->pb_bins64.p, (double *)y, s);
+            return FL_OK;
+        case FL_VAL_F32:
+            if (win) launch_win<float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, (float *)y, s);
+            else launch_pb<float, float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, s);
+            return FL_OK;
+        case FL_VAL_BF16:
+            if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, (float *)y, s);
+            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, s);
+            return FL_OK;
+        case FL_VAL_BF16X2:
+            if (win) launch_win<uint16_t, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, (double *)y, s);
+            else launch_pb<uint16_t, float, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, A->pb_bins32.p, (double *)y, s);
+            return FL_OK;
+        default:
+            set_error("fl_spmv: unknown value kind %d", kind);
+            return FL_ERR_INVALID;
+    }
+}
+
+}  // namespace fl

@@ -77,28 +77,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dropin(int T_, const int *__res
 }
 
 // -------------------------------------------------------------------------------- FAST
-// Level-walking tile iterator (wave-uniform state).
-struct LevelWalk {
-    int k, col, tall, chunks, nnzoff, j, e;
-};
-
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __forceinline__ void level_load(LevelWalk &s, int k, int n, const int *__restrict__ lvl_col,
-                                           const int *__restrict__ lvl_tall, const int *__restrict__ nnzoff) {
-    s.k = k;
-    s.col = uni(lvl_col[k]);
-    s.tall = uni(lvl_tall[k]);
-    s.nnzoff = uni(nnzoff[k]);
-    s.chunks = (n - s.col + 63) >> 6;
-}
-
 template <typename V, typename X, typename Y, int U, bool SCAN, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void k_spmv_fast64(int T_, int n, int groups, const int *__restrict__ lvl_col,
-                                                        const int *__restrict__ lvl_tall,
-                                                        const int *__restrict__ tileoff,
-                                                        const int *__restrict__ nnzoff,
-                                                        const int *__restrict__ group_level,
+__global__ __launch_bounds__(kBlock) void k_spmv_fast64(int T_, int groups, LevelTab lt,
                                                         const int *__restrict__ sortrowidx,
                                                         const V *__restrict__ val, const V *__restrict__ val_lo,
                                                         const X *__restrict__ x, Y *__restrict__ y,
@@ -110,26 +90,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv_fast64(int T_, int n, int group
         const int t0 = gi * 64;
         const int t1 = min(t0 + 64, T_);
         LevelWalk s;
-        level_load(s, uni(group_level[gi]), n, lvl_col, lvl_tall, nnzoff);
-        {
-            const int local = t0 - uni(tileoff[s.k]);
-            s.j = local / s.tall;
-            s.e = local - s.j * s.tall;
-        }
+        s.seek(lt, t0);
         for (int t = t0; t < t1; t += U) {
             int ptr[U], cx[U], len[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (t + u < t1) {
-                    const int width = n - s.col;
-                    len[u] = (s.j == s.chunks - 1) ? width - (s.chunks - 1) * 64 : 64;
-                    ptr[u] = s.nnzoff + s.j * s.tall * 64 + s.e * len[u];
-                    cx[u] = s.col + 64 * s.j;
-                    if (++s.e == s.tall) {
-                        s.e = 0;
-                        if (++s.j == s.chunks && t + u + 1 < t1)
-                            { level_load(s, s.k + 1, n, lvl_col, lvl_tall, nnzoff); s.j = 0; }
-                    }
+                    s.next(lt, ptr[u], cx[u], len[u], t + u + 1 < t1);
                 } else {
                     len[u] = 0;
                     ptr[u] = 0;
@@ -272,14 +239,19 @@ static void launch_fast(const fl_matrix *A, const V *val, const V *val_lo, const
     const int groups = (A->tilenum + 63) / 64;
     constexpr int U = 4;
     const unsigned g = grid_for((long long)groups * 64);
-    k_spmv_fast64<V, X, Y, U, true, SPLIT><<<g, kBlock, 0, s>>>(A->tilenum, A->n, groups, A->lvl_col.p, A->lvl_tall.p,
-                                                                A->lvl_tileoff.p, A->lvl_nnzoff.p, A->group_level.p,
-                                                                A->sortrowidx.p, val, val_lo, x, y, y_lo);
+    k_spmv_fast64<V, X, Y, U, true, SPLIT><<<g, kBlock, 0, s>>>(A->tilenum, groups, level_tab(A), A->sortrowidx.p, val,
+                                                                val_lo, x, y, y_lo);
     FL_LAUNCH_CHECK();
 }
 
-int spmv(const fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, hipStream_t s) {
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, hipStream_t s);
+
+int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, hipStream_t s) {
     if (A->tilenum == 0) return FL_OK;
+    if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_PB) {
+        if (A->W != 64) { set_error("fl_spmv: batched schedules need a 64-wide tile layout"); return FL_ERR_INVALID; }
+        return spmv_batched(A, kernel, kind, x, y, s);
+    }
     if (kernel == FL_KERNEL_DROPIN) {
         if (kind == FL_VAL_F64) {
             if (A->W == 32) launch_dropin<double, 32>(A, (const double *)x, (double *)y, s);

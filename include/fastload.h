@@ -182,12 +182,30 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
  *     several tiles per wave in flight. */
 #define FL_KERNEL_DROPIN 0
 #define FL_KERNEL_FAST 1
+/*   FL_KERNEL_WIN: batches of consecutive tiles per workgroup; the distinct rows of a batch are
+ *     compacted to LDS slots (2-byte slot per entry instead of a 4-byte row), products are summed
+ *     per slot in FP64 LDS atomics and each slot is flushed with one global atomic. For rows with
+ *     locality (stencils, bands, blocks).
+ *   FL_KERNEL_PB: two-phase propagation blocking for rows without locality: phase 1 stages the
+ *     batch's products in LDS by row band and copies them band-major; phase 2 sums each band of
+ *     rows in FP64 LDS and updates y once per row. No global atomics. */
+#define FL_KERNEL_WIN 2
+#define FL_KERNEL_PB 3
 
 /* y += A * sortx (y must be zeroed by the caller for y = A*x, as src/main.cu:288-313 does).
+ * FL_KERNEL_WIN/FL_KERNEL_PB need fl_matrix_prepare_schedule first; they write FL_VAL_BF16X2
+ * results to a double y directly and need no work buffer.
  * val_kind selects the value copy (see fl_matrix_prepare_values); pointer types:
  *   FL_VAL_F64: x,y double.  FL_VAL_F32/FL_VAL_BF16: x,y float.
  *   FL_VAL_BF16X2: x float; y double (recombined); needs d_work of 2*m floats (zeroed inside). */
-int fl_spmv(const fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work, void *stream);
+int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work, void *stream);
+
+/* Builds (once per parameter set) the preprocessing of a batched schedule on a 64-wide matrix:
+ *   kernel FL_KERNEL_WIN: batch_tiles in {64, 128, 256} (LDS = batch_tiles*512 bytes);
+ *   kernel FL_KERNEL_PB : batch_tiles as above, band_rows in [1024, 16384] (LDS = band_rows*8 bytes).
+ * The PB schedule keeps a product buffer of nnz floats (doubles in FP64 mode) allocated on its
+ * first use. */
+int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int band_rows, void *stream);
 
 #ifdef __cplusplus
 }

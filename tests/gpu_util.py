@@ -8,9 +8,11 @@ import fastload_amd as fl
 from fastload_amd import DeviceArray
 
 
-def device_y(M: "fl.FastLoadMatrix", x64, kernel, kind):
+def device_y(M: "fl.FastLoadMatrix", x64, kernel, kind, bt=256, R=16384):
     """y = A x through fl_permute_x_* + fl_spmv; returns float64 host y."""
     m = M.m
+    if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+        M.prepare_schedule(kernel, bt, R)
     if kind == fl.VAL_F64:
         dx = DeviceArray.from_host(np.asarray(x64, np.float64))
         dsx = DeviceArray(M.n, np.float64)
@@ -24,7 +26,7 @@ def device_y(M: "fl.FastLoadMatrix", x64, kernel, kind):
     M.permute_x(dx, dsx)
     if kind == fl.VAL_BF16X2:
         dy = DeviceArray.zeros(m, np.float64)
-        work = DeviceArray(2 * max(m, 1), np.float32)
+        work = DeviceArray(2 * max(m, 1), np.float32) if kernel == fl.KERNEL_FAST else None
         M.spmv(dsx, dy, kernel, kind, work=work)
         return dy.to_host()
     dy = DeviceArray.zeros(m, np.float32)

@@ -124,7 +124,9 @@ def test_device_layout_matches_reference_golden(gpu):
 KERNELS = [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_DROPIN, fl.VAL_F64, 64), (fl.KERNEL_DROPIN, fl.VAL_F32, 32),
            (fl.KERNEL_DROPIN, fl.VAL_F32, 64), (fl.KERNEL_FAST, fl.VAL_F64, 64), (fl.KERNEL_FAST, fl.VAL_F32, 64),
            (fl.KERNEL_FAST, fl.VAL_BF16, 64), (fl.KERNEL_FAST, fl.VAL_BF16X2, 64)]
-KIDS = [f"{'dropin' if k == 0 else 'fast'}-{['f64', 'f32', 'bf16', 'bf16x2'][v]}-w{w}" for k, v, w in KERNELS]
+KERNELS += [(k, v, 64) for k in (fl.KERNEL_WIN, fl.KERNEL_PB) for v in (fl.VAL_F64, fl.VAL_F32, fl.VAL_BF16, fl.VAL_BF16X2)]
+KNAMES = ["dropin", "fast", "win", "pb"]
+KIDS = [f"{KNAMES[k]}-{['f64', 'f32', 'bf16', 'bf16x2'][v]}-w{w}" for k, v, w in KERNELS]
 
 
 @pytest.mark.parametrize("kernel,kind,W", KERNELS, ids=KIDS)
@@ -166,3 +168,18 @@ def test_kat_9x9_gpu(gpu):
                 continue
             y = device_y(M, np.ones(9), kernel, kind)
             assert y.tolist() == kat["row_sums"], (W, kernel, kind)
+
+
+@pytest.mark.parametrize("kernel", [fl.KERNEL_WIN, fl.KERNEL_PB], ids=["win", "pb"])
+@pytest.mark.parametrize("bt,R", [(64, 1024), (128, 2048), (256, 4096), (256, 16384)])
+def test_batched_schedule_params(gpu, kernel, bt, R):
+    """Multi-batch / multi-band coverage on matrices small enough to check against the oracle."""
+    for name, m, n, cp, ri, v, x in CASES:
+        if name in ("all_empty",):
+            continue
+        M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+        y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
+        for kind in (fl.VAL_F64, fl.VAL_F32):
+            y = device_y(M, x, kernel, kind, bt=bt, R=R)
+            err = rel_err(y, y_or, scale)
+            assert err <= TOL[kind], f"{name} bt={bt} R={R} kind={kind}: {err:.3e}"
