@@ -53,6 +53,15 @@ VARIANTS = {
     "pb_bf16x2": (64, 3, 3, 4, 256, 16384),
     "pb_bf16": (64, 3, 2, 2, 256, 16384),
     "pb_f64": (64, 3, 0, 8, 256, 16384),
+    # schedule-parameter sweep points
+    "win128_f32": (64, 2, 1, 4, 128, 0),
+    "win64_f32": (64, 2, 1, 4, 64, 0),
+    "win128_bf16x2": (64, 2, 3, 4, 128, 0),
+    "win128_bf16": (64, 2, 2, 2, 128, 0),
+    "win128_f64": (64, 2, 0, 8, 128, 0),
+    "pb128_f32": (64, 3, 1, 4, 128, 16384),
+    "pb8k_f32": (64, 3, 1, 4, 256, 8192),
+    "pb128_8k_f32": (64, 3, 1, 4, 128, 8192),
 }
 
 

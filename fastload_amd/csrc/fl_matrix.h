@@ -27,9 +27,14 @@ struct fl_matrix {
     // FAST kernel: level index of the first tile of every 64-tile group
     fl::DevBuf<int> group_level;
 
-    // Batched schedules (W = 64): a batch is `bt` consecutive tiles (<= bt*64 entries).
+    // Batched schedules (W = 64): the tiles are re-ordered by 256-column bucket of their first
+    // sorted column (stable), so a batch of `bt` consecutive re-ordered tiles reads a narrow
+    // window of x; ord_ptr/ord_cx give each re-ordered tile's entry offset and first column
+    // (len = min(64, n - cx)).
     int bt = 0, nbatches = 0;
-    fl::DevBuf<int> batch_of;  // [nnz] batch index of every entry (tile order), built on demand
+    fl::DevBuf<int> ord_ptr, ord_cx;
+    fl::DevBuf<int> batch_of;     // [nnz] batch index of every entry (tile order), built on demand
+    fl::DevBuf<int> batch_start;  // [nbatches+1] prefix of batch entry counts
     // WIN: rows of each batch compacted to LDS slots.
     //   win_slot[g]  slot of entry g inside its batch; win_rows[win_off[b] + s] = row of slot s
     bool win_ready = false;

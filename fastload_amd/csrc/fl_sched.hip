@@ -3,9 +3,11 @@
 // global atomics -- one per segment in the reference kernel (src/FastLoad_GPU.h:186,224,267)
 // -- are replaced by LDS work.
 //
-// A batch is `bt` consecutive tiles (<= bt*64 entries), handled by one 1024-thread workgroup;
-// each of its 16 waves walks bt/16 consecutive tiles with the level table (fl_walk.h), so x
-// is read exactly as in FastLoad: 64 consecutive sorted columns per tile.
+// The tiles are re-ordered (stable) by the 256-column bucket of their first sorted column, so
+// tiles of different levels that read the same part of x run in the same batch. A batch is `bt`
+// consecutive re-ordered tiles (<= bt*64 entries), handled by one 1024-thread workgroup; each
+// of its 16 waves processes bt/16 tiles, reading 64 consecutive sorted x values per tile as in
+// FastLoad, from a window of x that stays in L1/L2 instead of being re-streamed per level.
 //
 // WIN (rows with locality: stencils, bands, blocks): the distinct rows of a batch are
 //   compacted to LDS slots at preprocessing time; entries store a 2-byte slot instead of a
@@ -30,12 +32,37 @@ static unsigned sg(long long count) { return std::max(1u, std::min<unsigned>(gri
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (count); i += (long long)gridDim.x * blockDim.x)
 
 // ------------------------------------------------------------------------ preprocessing
-__global__ void k_batch_of(int T, int bt, const int *__restrict__ tile_ptr, const int *__restrict__ tile_len,
+// re-ordered tile i (original tile order[i]) -> its entries belong to batch i / bt
+__global__ void k_batch_of(int T, int bt, int n, const int *__restrict__ ord_ptr, const int *__restrict__ ord_cx,
                            int *__restrict__ batch_of) {
     GRID_STRIDE(g, (long long)T * 64) {
-        const long long t = g >> 6;
+        const long long i = g >> 6;
         const int l = (int)(g & 63);
-        if (l < tile_len[t]) batch_of[tile_ptr[t] + l] = (int)(t / bt);
+        if (l < min(64, n - ord_cx[i])) batch_of[ord_ptr[i] + l] = (int)(i / bt);
+    }
+}
+
+__global__ void k_tile_bucket(const int *__restrict__ tile_colidx, int T, uint32_t *__restrict__ key) {
+    GRID_STRIDE(t, T) key[t] = (uint32_t)(tile_colidx[t] >> 8);
+}
+
+__global__ void k_tile_reorder(const int *__restrict__ order, const int *__restrict__ tile_ptr,
+                               const int *__restrict__ tile_colidx, int T, int *__restrict__ ord_ptr,
+                               int *__restrict__ ord_cx) {
+    GRID_STRIDE(i, T) {
+        const int t = order[i];
+        ord_ptr[i] = tile_ptr[t];
+        ord_cx[i] = tile_colidx[t];
+    }
+}
+
+// entry count of each batch (sum of its re-ordered tiles' lengths)
+__global__ void k_batch_sizes(int T, int bt, int nb, int n, const int *__restrict__ ord_cx, int *__restrict__ sizes) {
+    GRID_STRIDE(b, nb) {
+        int s = 0;
+        const int i1 = min((int)(b + 1) * bt, T);
+        for (int i = (int)b * bt; i < i1; ++i) s += min(64, n - ord_cx[i]);
+        sizes[b] = s;
     }
 }
 
@@ -50,6 +77,11 @@ __global__ void k_key_gather(const int *__restrict__ src, const int *__restrict_
 
 __global__ void k_key_copy(const int *__restrict__ src, uint32_t *__restrict__ key, int count, int div) {
     GRID_STRIDE(i, count) key[i] = (uint32_t)(src[i] / div);
+}
+
+__global__ void k_key_gather_div(const int *__restrict__ src, const int *__restrict__ order, int div,
+                                 uint32_t *__restrict__ key, int count) {
+    GRID_STRIDE(i, count) key[i] = (uint32_t)(src[order[i]] / div);
 }
 
 __global__ void k_key_batch_band(const int *__restrict__ batch_of, const int *__restrict__ row, int R, int nbands,
@@ -93,8 +125,16 @@ __global__ void k_pb_dest(const int *__restrict__ orderA, const int *__restrict_
     }
 }
 
-__global__ void k_count_keys(const uint32_t *__restrict__ k, int count, int *__restrict__ counts) {
-    GRID_STRIDE(i, count) atomicAdd(&counts[k[i]], 1);
+// start[b] = first position of key b in the sorted keys (lower bound); start[nb] = count.
+__global__ void k_key_bounds(const uint32_t *__restrict__ k, int count, int nb, int *__restrict__ start) {
+    GRID_STRIDE(b, (long long)nb + 1) {
+        int lo = 0, hi = count;
+        while (lo < hi) {
+            const int mid = (int)(((long long)lo + hi) >> 1);
+            if (k[mid] < (uint32_t)b) lo = mid + 1; else hi = mid;
+        }
+        start[b] = lo;
+    }
 }
 
 __global__ void k_pb_heads(const uint32_t *__restrict__ kb, int *__restrict__ head, int count) {
@@ -102,13 +142,13 @@ __global__ void k_pb_heads(const uint32_t *__restrict__ kb, int *__restrict__ he
 }
 
 __global__ void k_pb_runs(const uint32_t *__restrict__ kb, const int *__restrict__ orderB, const int *__restrict__ head,
-                          const int *__restrict__ runidx, const int *__restrict__ dest, const int *__restrict__ tile_ptr,
-                          int bt, int nbands, uint16_t *__restrict__ lslot, int *__restrict__ run_lstart,
-                          int *__restrict__ run_dest, int *__restrict__ run_pos, int *__restrict__ run_off,
-                          int count) {
+                          const int *__restrict__ runidx, const int *__restrict__ dest,
+                          const int *__restrict__ batch_start, int nbands, uint16_t *__restrict__ lslot,
+                          int *__restrict__ run_lstart, int *__restrict__ run_dest, int *__restrict__ run_pos,
+                          int *__restrict__ run_off, int count) {
     GRID_STRIDE(i, count) {
         const int batch = (int)(kb[i] / (uint32_t)nbands);
-        const int bstart = tile_ptr[batch * bt];
+        const int bstart = batch_start[batch];
         const int g = orderB[i];
         lslot[g] = (uint16_t)(i - bstart);
         if (head[i]) {
@@ -123,6 +163,18 @@ __global__ void k_pb_runs(const uint32_t *__restrict__ kb, const int *__restrict
 
 __global__ void k_run_len(const int *__restrict__ run_pos, int nruns, int total, int *__restrict__ run_len) {
     GRID_STRIDE(r, nruns) run_len[r] = ((r + 1 < nruns) ? run_pos[r + 1] : total) - run_pos[r];
+}
+
+// Every entry's destination must be its run's destination plus its offset inside the run (the
+// phase-1 copy moves whole runs).
+__global__ void k_check_runs(const int *__restrict__ orderB, const int *__restrict__ head,
+                             const int *__restrict__ runidx, const int *__restrict__ run_pos,
+                             const int *__restrict__ run_dest, const int *__restrict__ dest, int count,
+                             int *__restrict__ flag) {
+    GRID_STRIDE(i, count) {
+        const int r = runidx[i] + head[i] - 1;
+        if (dest[orderB[i]] != run_dest[r] + ((int)i - run_pos[r])) atomicOr(flag, 16);
+    }
 }
 
 // Validators: every index the kernels will use must be inside the buffers they assume.
@@ -142,17 +194,17 @@ __global__ void k_check_win(const int *__restrict__ batch_of, const uint16_t *__
 }
 
 __global__ void k_check_pb(const int *__restrict__ batch_of, const uint16_t *__restrict__ lslot,
-                           const int *__restrict__ tile_ptr, int bt, int T, int nnz,
+                           const int *__restrict__ batch_start, int cap, int nnz,
                            const int *__restrict__ run_off, const int *__restrict__ run_lstart,
                            const int *__restrict__ run_dest, const int *__restrict__ run_len, int nb,
                            int *__restrict__ flag) {
     GRID_STRIDE(g, nnz) {
         const int b = batch_of[g];
-        const int bsize = tile_ptr[min((b + 1) * bt, T)] - tile_ptr[b * bt];
-        if ((int)lslot[g] >= bsize) atomicOr(flag, 4);
+        const int bsize = batch_start[b + 1] - batch_start[b];
+        if (bsize > cap || (int)lslot[g] >= bsize) atomicOr(flag, 4);
     }
     GRID_STRIDE(b, nb) {
-        const int bsize = tile_ptr[min((int)(b + 1) * bt, T)] - tile_ptr[b * bt];
+        const int bsize = batch_start[b + 1] - batch_start[b];
         for (int r = run_off[b]; r < run_off[b + 1]; ++r)
             if (run_len[r] <= 0 || run_lstart[r] < 0 || run_lstart[r] + run_len[r] > bsize || run_dest[r] < 0 ||
                 run_dest[r] + run_len[r] > nnz)
@@ -173,16 +225,38 @@ static void write1(int *d, int v, hipStream_t s) {
 }
 
 static void ensure_batches(fl_matrix *A, int bt, hipStream_t s) {
+    const int T = A->tilenum;
+    if (!A->ord_ptr.p) {
+        // stable sort of the tiles by 256-column bucket of their first sorted column
+        A->ord_ptr.alloc(std::max(T, 1));
+        A->ord_cx.alloc(std::max(T, 1));
+        if (T > 0) {
+            DevBuf<uint32_t> key(T);
+            DevBuf<int> order(T);
+            k_tile_bucket<<<sg(T), kBlock, 0, s>>>(A->tile_colidx.p, T, key.p);
+            k_iota2<<<sg(T), kBlock, 0, s>>>(order.p, T);
+            FL_LAUNCH_CHECK();
+            radix_sort_pairs(key.p, order.p, T, bits_for((uint32_t)std::max((A->n - 1) >> 8, 0)), s);
+            k_tile_reorder<<<sg(T), kBlock, 0, s>>>(order.p, A->tile_ptr.p, A->tile_colidx.p, T, A->ord_ptr.p,
+                                                    A->ord_cx.p);
+            FL_LAUNCH_CHECK();
+            FL_HIP(hipStreamSynchronize(s));
+        }
+    }
     if (A->bt == bt && A->batch_of.p) return;
     A->bt = bt;
-    A->nbatches = (A->tilenum + bt - 1) / bt;
+    A->nbatches = (T + bt - 1) / bt;
     A->win_ready = A->pb_ready = false;
     A->batch_of.alloc(std::max(A->nnz, 1));
-    if (A->tilenum > 0) {
-        k_batch_of<<<sg((long long)A->tilenum * 64), kBlock, 0, s>>>(A->tilenum, bt, A->tile_ptr.p, A->tile_len.p,
-                                                                     A->batch_of.p);
+    A->batch_start.alloc((size_t)A->nbatches + 1);
+    DevBuf<int> sizes(std::max(A->nbatches, 1));
+    if (T > 0) {
+        k_batch_of<<<sg((long long)T * 64), kBlock, 0, s>>>(T, bt, A->n, A->ord_ptr.p, A->ord_cx.p, A->batch_of.p);
+        k_batch_sizes<<<sg(A->nbatches), kBlock, 0, s>>>(T, bt, A->nbatches, A->n, A->ord_cx.p, sizes.p);
         FL_LAUNCH_CHECK();
     }
+    exclusive_scan_i32(sizes.p, A->batch_start.p, A->nbatches, s);
+    FL_HIP(hipStreamSynchronize(s));
 }
 
 void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
@@ -253,17 +327,19 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
         return;
     }
     DevBuf<uint32_t> key(nnz);
-    DevBuf<int> order(nnz), dest(nnz), head(nnz), runidx((size_t)nnz + 1), counts(nbands);
-    // A: band-major order (band, g)
-    k_key_copy<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, key.p, nnz, R);
+    DevBuf<int> order(nnz), dest(nnz), head(nnz), runidx((size_t)nnz + 1);
+    // A: band-major order (band, batch, g): stable by batch, then stable by band, so each
+    // (batch, band) run occupies one contiguous range of the product array
+    k_key_copy<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, key.p, nnz, 1);
     k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(nb - 1, 0)), s);
+    k_key_gather_div<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, order.p, R, key.p, nnz);
     FL_LAUNCH_CHECK();
     radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)(nbands - 1)), s);
     k_pb_dest<<<sg(nnz), kBlock, 0, s>>>(order.p, A->sortrowidx.p, R, dest.p, A->pb_rowlocal.p, nnz);
-    FL_HIP(hipMemsetAsync(counts.p, 0, sizeof(int) * nbands, s));
-    k_count_keys<<<sg(nnz), kBlock, 0, s>>>(key.p, nnz, counts.p);
+    k_key_bounds<<<sg((long long)nbands + 1), kBlock, 0, s>>>(key.p, nnz, nbands, A->pb_band_start.p);
     FL_LAUNCH_CHECK();
-    exclusive_scan_i32(counts.p, A->pb_band_start.p, nbands, s);
     // B: (batch, band, g) order -> LDS staging slots and runs
     k_key_batch_band<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->sortrowidx.p, R, nbands, key.p, nnz);
     k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
@@ -278,7 +354,7 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     A->pb_run_dest.alloc(nruns);
     A->pb_run_len.alloc(nruns);
     DevBuf<int> run_pos(nruns);
-    k_pb_runs<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, head.p, runidx.p, dest.p, A->tile_ptr.p, A->bt, nbands,
+    k_pb_runs<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, head.p, runidx.p, dest.p, A->batch_start.p, nbands,
                                          A->pb_lslot.p, A->pb_run_lstart.p, A->pb_run_dest.p, run_pos.p,
                                          A->pb_run_off.p, nnz);
     k_run_len<<<sg(nruns), kBlock, 0, s>>>(run_pos.p, nruns, nnz, A->pb_run_len.p);
@@ -286,9 +362,11 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     write1(A->pb_run_off.p + nb, nruns, s);
     DevBuf<int> flag(1);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
-    k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->tile_ptr.p, A->bt, A->tilenum, nnz,
+    k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->batch_start.p, A->bt * 64, nnz,
                                           A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, nb,
                                           flag.p);
+    k_check_runs<<<sg(nnz), kBlock, 0, s>>>(order.p, head.p, runidx.p, run_pos.p, A->pb_run_dest.p, dest.p, nnz,
+                                            flag.p);
     FL_LAUNCH_CHECK();
     const int total = read1(A->pb_band_start.p + nbands, s);
     if (read1(flag.p, s) || total != nnz) {
@@ -334,38 +412,63 @@ __device__ __forceinline__ double product(const V *__restrict__ val, const V *__
 __device__ __forceinline__ void gatomic(float *p, double v) { unsafeAtomicAdd(p, (float)v); }
 __device__ __forceinline__ void gatomic(double *p, double v) { unsafeAtomicAdd(p, v); }
 
-// Per-wave tile loop over this batch: U tiles in flight, fn(g, xi) for every active lane.
+// Re-ordered tiles of the batched schedules: tile i starts at entry ptr[i] and covers sorted
+// columns [cx[i], cx[i] + min(64, n - cx[i])).
+struct OrdTiles {
+    const int *__restrict__ ptr;
+    const int *__restrict__ cx;
+    int n, T;
+};
+
+// Per-wave tile loop over this batch: the wave's (<= 16) tiles' metadata is loaded once (lane j
+// holds tile j), then U tiles at a time are handed to fn(ptr[], cx[], len[], lane).
 template <int U, typename F>
-__device__ __forceinline__ void batch_tiles(const LevelTab &lt, int T_, int bt, int batch, F &&fn) {
+__device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batch, F &&fn) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tpw = bt / (kBigBlock / 64);
     const int t0 = batch * bt + wave * tpw;
-    const int t1 = min(t0 + tpw, T_);
+    const int t1 = min(t0 + tpw, ot.T);
     if (t0 >= t1) return;
-    LevelWalk s;
-    s.seek(lt, t0);
+    const int mp = (lane < t1 - t0) ? ot.ptr[t0 + lane] : 0;
+    const int mc = (lane < t1 - t0) ? ot.cx[t0 + lane] : 0;
     for (int t = t0; t < t1; t += U) {
         int ptr[U], cx[U], len[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (t + u < t1) s.next(lt, ptr[u], cx[u], len[u], t + u + 1 < t1);
-            else { ptr[u] = 0; cx[u] = 0; len[u] = 0; }
+            if (t + u < t1) {
+                ptr[u] = __builtin_amdgcn_readlane(mp, t + u - t0);
+                cx[u] = __builtin_amdgcn_readlane(mc, t + u - t0);
+                len[u] = min(64, ot.n - cx[u]);
+            } else {
+                ptr[u] = 0;
+                cx[u] = 0;
+                len[u] = 0;
+            }
         }
         fn(ptr, cx, len, lane);
     }
 }
 
-template <typename V, typename X, typename Y, bool SPLIT, int U>
-__global__ __launch_bounds__(kBigBlock) void k_win(int T_, int bt, LevelTab lt, const uint16_t *__restrict__ slot,
+template <typename V, typename X, typename Y, bool SPLIT, int U, int BT>
+__global__ __launch_bounds__(kBigBlock) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
                                                    const V *__restrict__ val, const V *__restrict__ val_lo,
                                                    const X *__restrict__ x, const int *__restrict__ win_rows,
                                                    const int *__restrict__ win_off, Y *__restrict__ y) {
     extern __shared__ double acc[];
     const int b = blockIdx.x;
     const int s0 = win_off[b], ns = win_off[b + 1] - s0;
+    // prefetch the rows this thread will flush (ns <= BT*64 = K * kBigBlock) so the flush after
+    // the barrier issues its atomics without waiting on loads
+    constexpr int K = BT * 64 / kBigBlock;
+    int frow[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * kBigBlock;
+        frow[k] = (i < ns) ? win_rows[s0 + i] : 0;
+    }
     for (int i = threadIdx.x; i < ns; i += kBigBlock) acc[i] = 0.0;
     __syncthreads();
-    batch_tiles<U>(lt, T_, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+    batch_tiles<U>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
         int sl[U];
         double p[U];
 #pragma unroll
@@ -380,11 +483,15 @@ __global__ __launch_bounds__(kBigBlock) void k_win(int T_, int bt, LevelTab lt, 
             if (lane < len[u]) atomicAdd(&acc[sl[u]], p[u]);
     });
     __syncthreads();
-    for (int i = threadIdx.x; i < ns; i += kBigBlock) gatomic(&y[win_rows[s0 + i]], acc[i]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * kBigBlock;
+        if (i < ns) gatomic(&y[frow[k]], acc[i]);
+    }
 }
 
 template <typename V, typename X, typename P, bool SPLIT, int U>
-__global__ __launch_bounds__(kBigBlock) void k_pb1(int T_, int bt, LevelTab lt, const uint16_t *__restrict__ lslot,
+__global__ __launch_bounds__(kBigBlock) void k_pb1(OrdTiles ot, int bt, const uint16_t *__restrict__ lslot,
                                                    const V *__restrict__ val, const V *__restrict__ val_lo,
                                                    const X *__restrict__ x, const int *__restrict__ run_off,
                                                    const int *__restrict__ run_lstart, const int *__restrict__ run_dest,
@@ -392,7 +499,20 @@ __global__ __launch_bounds__(kBigBlock) void k_pb1(int T_, int bt, LevelTab lt, 
     extern __shared__ double stage_raw[];
     P *stage = reinterpret_cast<P *>(stage_raw);
     const int b = blockIdx.x;
-    batch_tiles<U>(lt, T_, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+    const int wave = threadIdx.x >> 6;
+    // Wave w copies runs r0 + w + 16*j. Prefetch the metadata of its first 64 runs (lane j holds
+    // run j) before the tile loop so the copy loop only reads registers.
+    const int r0 = run_off[b], r1 = run_off[b + 1];
+    int mlen = 0, mls = 0, mdst = 0;
+    {
+        const int r = r0 + wave + 16 * (int)(threadIdx.x & 63);
+        if (r < r1) {
+            mlen = run_len[r];
+            mls = run_lstart[r];
+            mdst = run_dest[r];
+        }
+    }
+    batch_tiles<U>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
         int sl[U];
         double p[U];
 #pragma unroll
@@ -407,42 +527,89 @@ __global__ __launch_bounds__(kBigBlock) void k_pb1(int T_, int bt, LevelTab lt, 
             if (lane < len[u]) stage[sl[u]] = (P)p[u];
     });
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = run_off[b], r1 = run_off[b + 1];
-    for (int r = r0 + wave; r < r1; r += kBigBlock / 64) {
-        const int len = run_len[r], ls = run_lstart[r], d = run_dest[r];
-        for (int i = lane; i < len; i += 64) bins[d + i] = stage[ls + i];
+    const int lane = threadIdx.x & 63;
+    for (int base = 0;; base += 64) {
+        for (int j = 0; j < 64; ++j) {
+            const int len = __builtin_amdgcn_readlane(mlen, j);
+            if (len == 0) break;  // past the last run of this wave (every real run has len >= 1)
+            const int ls = __builtin_amdgcn_readlane(mls, j), d = __builtin_amdgcn_readlane(mdst, j);
+            for (int i = lane; i < len; i += 64) bins[d + i] = stage[ls + i];
+        }
+        if (r0 + wave + 16 * (base + 64) >= r1) break;
+        const int r = r0 + wave + 16 * (base + 64 + lane);
+        mlen = mls = mdst = 0;
+        if (r < r1) {
+            mlen = run_len[r];
+            mls = run_lstart[r];
+            mdst = run_dest[r];
+        }
     }
 }
+
+template <typename P>
+struct Pair;
+template <>
+struct Pair<float> {
+    using T = float2;
+};
+template <>
+struct Pair<double> {
+    using T = double2;
+};
 
 template <typename P, typename Y, int U>
 __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
                                                    const P *__restrict__ bins, const uint16_t *__restrict__ rowlocal,
                                                    Y *__restrict__ y) {
     extern __shared__ double acc[];
+    using P2 = typename Pair<P>::T;
     const int b = blockIdx.x;
     for (int i = threadIdx.x; i < R; i += kBigBlock) acc[i] = 0.0;
     __syncthreads();
     const int i0 = band_start[b], i1 = band_start[b + 1];
-    int i = i0 + threadIdx.x;
-    for (; i + (U - 1) * kBigBlock < i1; i += U * kBigBlock) {
-        P v[U];
-        int rl[U];
+    // pairs of entries: one 2-wide product load and one 4-byte load of two 16-bit rows
+    const int a0 = i0 + (i0 & 1);
+    if (a0 != i0 && threadIdx.x == 0 && i0 < i1) atomicAdd(&acc[rowlocal[i0]], (double)bins[i0]);
+    const int np = (i1 > a0) ? (i1 - a0) >> 1 : 0;
+    const P2 *__restrict__ b2 = reinterpret_cast<const P2 *>(bins + a0);
+    const uint32_t *__restrict__ r2 = reinterpret_cast<const uint32_t *>(rowlocal + a0);
+    int k = threadIdx.x;
+    for (; k + (U - 1) * kBigBlock < np; k += U * kBigBlock) {
+        P2 v[U];
+        uint32_t rl[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            v[u] = bins[i + u * kBigBlock];
-            rl[u] = rowlocal[i + u * kBigBlock];
+            v[u] = b2[k + u * kBigBlock];
+            rl[u] = r2[k + u * kBigBlock];
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) atomicAdd(&acc[rl[u]], (double)v[u]);
+        for (int u = 0; u < U; ++u) {
+            atomicAdd(&acc[rl[u] & 0xffffu], (double)v[u].x);
+            atomicAdd(&acc[rl[u] >> 16], (double)v[u].y);
+        }
     }
-    for (; i < i1; i += kBigBlock) atomicAdd(&acc[rowlocal[i]], (double)bins[i]);
+    for (; k < np; k += kBigBlock) {
+        const P2 v = b2[k];
+        const uint32_t rl = r2[k];
+        atomicAdd(&acc[rl & 0xffffu], (double)v.x);
+        atomicAdd(&acc[rl >> 16], (double)v.y);
+    }
+    if (a0 + 2 * np < i1 && threadIdx.x == 0) atomicAdd(&acc[rowlocal[i1 - 1]], (double)bins[i1 - 1]);
     __syncthreads();
     const int r0 = b * R, rows = min(R, m - r0);
     for (int r = threadIdx.x; r < rows; r += kBigBlock) y[r0 + r] += (Y)acc[r];
 }
 
 // ------------------------------------------------------------------------------ launch
+static OrdTiles ord_tiles(const fl_matrix *A) {
+    OrdTiles ot;
+    ot.ptr = A->ord_ptr.p;
+    ot.cx = A->ord_cx.p;
+    ot.n = A->n;
+    ot.T = A->tilenum;
+    return ot;
+}
+
 template <typename K>
 static void allow_lds(K kern, size_t bytes) {
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -452,9 +619,10 @@ template <typename V, typename X, typename Y, bool SPLIT>
 static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x, Y *y, hipStream_t s) {
     constexpr int U = 4;
     const size_t lds = (size_t)A->bt * 64 * sizeof(double);
-    auto kern = k_win<V, X, Y, SPLIT, U>;
+    auto kern = (A->bt == 256) ? k_win<V, X, Y, SPLIT, U, 256>
+                : (A->bt == 128) ? k_win<V, X, Y, SPLIT, U, 128> : k_win<V, X, Y, SPLIT, U, 64>;
     allow_lds(kern, lds);
-    kern<<<A->nbatches, kBigBlock, lds, s>>>(A->tilenum, A->bt, level_tab(A), A->win_slot.p, val, lo, x, A->win_rows.p,
+    kern<<<A->nbatches, kBigBlock, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, lo, x, A->win_rows.p,
                                              A->win_off.p, y);
     FL_LAUNCH_CHECK();
 }
@@ -465,7 +633,7 @@ static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x,
     const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
     auto k1 = k_pb1<V, X, P, SPLIT, U>;
     allow_lds(k1, std::max<size_t>(lds1, 8));
-    k1<<<A->nbatches, kBigBlock, lds1, s>>>(A->tilenum, A->bt, level_tab(A), A->pb_lslot.p, val, lo, x,
+    k1<<<A->nbatches, kBigBlock, lds1, s>>>(ord_tiles(A), A->bt, A->pb_lslot.p, val, lo, x,
                                             A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p,
                                             bins);
     FL_LAUNCH_CHECK();

@@ -174,3 +174,15 @@ def test_fast_walk_reproduces_layout(seed):
     assert np.array_equal(ptr, t["tile_ptr"][:-1]) and np.array_equal(ln, t["tile_len"])
     assert np.array_equal(col, t["tile_colidx"])
     assert np.all(ptr + ln <= nnz) and np.all(col + ln <= n)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tile_len_closed_form_w64(seed):
+    """The batched schedules derive a tile's length from its first column alone:
+    len = min(64, n - colidx) for every 64-wide FastLoad tile."""
+    rng = np.random.default_rng(50 + seed)
+    n = int(rng.integers(1, 5000))
+    s = np.sort(rng.zipf(1.6, n).clip(0, 300)).astype(np.int32)
+    nnz = int(s.sum())
+    t = ob.transform(n, nnz, s, np.zeros(nnz, np.int32), np.zeros(nnz), 64)
+    assert np.array_equal(t["tile_len"], np.minimum(64, n - t["tile_colidx"]))
