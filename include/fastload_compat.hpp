@@ -1,0 +1,61 @@
+// fastload_compat.hpp -- the reference's C++ host API (skyler-ruiter/FastLoad, src/) with its
+// exact names and reference-parameter signatures, as inline wrappers over the C ABI in
+// fastload.h. A driver written against the reference (src/main.cu) compiles against this header
+// and links libfastload.so instead of the CUDA sources. As in the reference the wrappers return
+// void; failures are printed to stderr (the C ABI returns status codes, fastload.h).
+#ifndef FASTLOAD_COMPAT_HPP
+#define FASTLOAD_COMPAT_HPP
+
+#include <cstdio>
+
+#include "fastload.h"
+
+namespace fastload_compat_detail {
+inline void report(int rc, const char *what) {
+    if (rc < 0) std::fprintf(stderr, "FastLoad %s failed (%d): %s\n", what, rc, fl_last_error());
+}
+}  // namespace fastload_compat_detail
+
+// src/auxiliary.h:65-67
+inline int mmio_allinone(int *m, int *n, int *nnz, int *isSymmetric, int **csrrowptr, int **csrcolidx,
+                         double **csrval, char *filename) {
+    return fl_mmio_allinone(m, n, nnz, isSymmetric, csrrowptr, csrcolidx, csrval, filename);
+}
+
+// src/csr2csc_cuda.h:51-53
+inline void csr_to_csc(double &time_csr2csc, int m, int n, int nnz, const double *csr_val, const int *csr_row_ptr,
+                       const int *csr_col_ind, double *csc_val, int *csc_row_ind, int *csc_col_ptr, int *nnzpercol) {
+    fastload_compat_detail::report(fl_csr_to_csc(&time_csr2csc, m, n, nnz, csr_val, csr_row_ptr, csr_col_ind, csc_val,
+                                                 csc_row_ind, csc_col_ptr, nnzpercol),
+                                   "csr_to_csc");
+}
+
+// src/ColSort_GPU.h:46-58
+inline void ColSort(double &timeForSort, int rowA, int colA, int nnz, int *nnzPerCol, int *cscColPtr, int *cscRowIdx,
+                    double *cscVal, int *sortrowidxTmp, double *sortvalTmp, int *sortnnzTmp, double *x,
+                    double *sortx) {
+    fastload_compat_detail::report(fl_ColSort(&timeForSort, rowA, colA, nnz, nnzPerCol, cscColPtr, cscRowIdx, cscVal,
+                                              sortrowidxTmp, sortvalTmp, sortnnzTmp, x, sortx),
+                                   "ColSort");
+}
+
+// src/formatTransform_GPU.h:289-298
+inline void formatTransform(double &timeFormatTran, double &timeFormatclas, slide_matrix *matrix,
+                            int *sortrowidx_tmp, double *sortval_tmp, int *sortnnz_tmp, int nnz, int rowA, int colA,
+                            int &h_count) {
+    fastload_compat_detail::report(fl_formatTransform(&timeFormatTran, &timeFormatclas, matrix, sortrowidx_tmp,
+                                                      sortval_tmp, sortnnz_tmp, nnz, rowA, colA, &h_count),
+                                   "formatTransform");
+}
+
+// src/FastLoad_GPU.h:275-282 (note the colA, rowA order)
+inline void FastLoad_spmv(char *filename, slide_matrix *matrix, int nnz, int colA, int rowA, double *x, double *y,
+                          double *y_golden) {
+    fastload_compat_detail::report(fl_FastLoad_spmv(filename, matrix, nnz, colA, rowA, x, y, y_golden, nullptr),
+                                   "FastLoad_spmv");
+}
+
+// src/format.h:29-42
+inline void Tile_destroy(slide_matrix *matrix) { fl_Tile_destroy(matrix); }
+
+#endif  // FASTLOAD_COMPAT_HPP

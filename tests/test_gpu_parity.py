@@ -183,3 +183,17 @@ def test_batched_schedule_params(gpu, kernel, bt, R):
             y = device_y(M, x, kernel, kind, bt=bt, R=R)
             err = rel_err(y, y_or, scale)
             assert err <= TOL[kind], f"{name} bt={bt} R={R} kind={kind}: {err:.3e}"
+
+
+def test_reference_style_driver(gpu, tmp_path):
+    """The drop-in boundary from C++: a src/main.cu-style driver compiled against
+    include/fastload_compat.hpp (the reference's own function names and signatures)."""
+    import subprocess
+    from conftest import ROOT
+    exe = tmp_path / "drop_in_main"
+    subprocess.run(["g++", "-std=c++17", "-I", str(ROOT / "include"), str(ROOT / "tests" / "drop_in_main.cpp"),
+                    "-L", str(ROOT / "fastload_amd"), "-lfastload", f"-Wl,-rpath,{ROOT / 'fastload_amd'}", "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe), str(GOLDEN / "1138_bus.mtx")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Check FASTLOAD GPU PASS!" in r.stdout, r.stdout + r.stderr
