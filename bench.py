@@ -143,6 +143,104 @@ def run_workload(fl, name, m, n, cp, ri, v, x, variants, steps, warmup):
     return nnz, mats, res
 
 
+def run_batched(fl, count, seed, steps, warmup):
+    """Batched configuration: `count` square CSC matrices, n log-uniform in [1e3, 2e5], column nnz
+    Zipf(2.0) truncated at n, FP32, k = 1, 8, 32 right-hand sides. The batch runs as one
+    block-diagonal matrix (k = 1: PB schedule; k > 1: ROW kernel) captured in a hipGraph, against
+    `count` sequential per-matrix launches (plain drop-in FP32 kernel for k = 1, per-matrix ROW
+    kernel for k > 1)."""
+    from fastload_amd import DeviceArray, Event, Graph, Stream
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), count)).astype(int)
+    t0 = time.time()
+    mats = [fl.gen_matrix(fl.GEN_ZIPF, seed * 100000 + i, int(n), int(n), (2.0, int(n))) for i, n in enumerate(sizes)]
+    M_, N_, cp, ri, v, ro, co = fl.batch_assemble(mats)
+    nnz = int(cp[-1])
+    log(f"[batched] {count} matrices, rows {sizes.min()}..{sizes.max()}, sum rows {M_}, nnz {nnz} "
+        f"generated+assembled in {time.time() - t0:.1f}s")
+    s = Stream()
+    B = fl.FastLoadMatrix(M_, N_, cp, ri, v, width=64)
+    B.prepare(fl.VAL_F32)
+    B.prepare_schedule(fl.KERNEL_PB, 256, 16384)
+    B.prepare_rows()
+    res = {"matrices": count, "rows": int(M_), "nnz": nnz}
+
+    def timed(fn, reps):
+        for _ in range(warmup):
+            fn()
+        s.synchronize()
+        a, b = Event(), Event()
+        a.record(s)
+        for _ in range(reps):
+            fn()
+        b.record(s)
+        s.synchronize()
+        return a.elapsed_ms(b) / reps
+
+    # per-matrix objects for the sequential baseline
+    singles = []
+    for (m, n, c, r, vv) in mats:
+        P = fl.FastLoadMatrix(m, n, c, r, vv, width=32)
+        P.prepare(fl.VAL_F32)
+        Q = fl.FastLoadMatrix(m, n, c, r, vv, width=64)
+        Q.prepare_rows()
+        singles.append((P, Q))
+    log(f"[batched] per-matrix preprocessing done ({time.time() - t0:.1f}s)")
+    for k in (1, 8, 32):
+        X = rng.standard_normal((N_, k)).astype(np.float32)
+        dX = DeviceArray.from_host(X.ravel())
+        dY = DeviceArray(M_ * k, np.float32)
+        B_alg = nnz * 8 + N_ * k * 4 + M_ * k * 4
+        if k == 1:
+            dsx = DeviceArray(N_, np.float32)
+            B.permute_x(dX, dsx)
+
+            def batch_step():
+                fl.zero_device(dY, s)
+                B.spmv(dsx, dY, fl.KERNEL_PB, fl.VAL_F32, stream=s)
+        else:
+            def batch_step():
+                B.spmm(dX, dY, k, fl.VAL_F32, stream=s)
+        batch_step()
+        s.synchronize()
+        y_batch = dY.to_host()
+        g = Graph(s)
+        with g:
+            batch_step()
+        t_graph = timed(g.launch, steps)
+        t_eager = timed(batch_step, steps)
+        # sequential per-matrix launches
+        xs, ys, sxs = [], [], []
+        for i, (P, Q) in enumerate(singles):
+            xi = X[co[i]:co[i + 1]]
+            xs.append(DeviceArray.from_host(np.ascontiguousarray(xi).ravel()))
+            ys.append(DeviceArray(mats[i][0] * k, np.float32))
+            if k == 1:
+                sxs.append(DeviceArray(mats[i][1], np.float32))
+                P.permute_x(xs[-1], sxs[-1])
+
+        def seq_step():
+            for i, (P, Q) in enumerate(singles):
+                if k == 1:
+                    fl.zero_device(ys[i], s)
+                    P.spmv(sxs[i], ys[i], fl.KERNEL_DROPIN, fl.VAL_F32, stream=s)
+                else:
+                    Q.spmm(xs[i], ys[i], k, fl.VAL_F32, stream=s)
+        t_seq = timed(seq_step, max(steps // 10, 3))
+        y_seq = np.concatenate([y.to_host() for y in ys])
+        err = float(np.linalg.norm(y_batch.astype(np.float64) - y_seq) / max(np.linalg.norm(y_seq), 1e-300))
+        res[f"k{k}"] = dict(graph_ms=round(t_graph, 4), eager_ms=round(t_eager, 4), sequential_ms=round(t_seq, 4),
+                            speedup_vs_sequential=round(t_seq / t_graph, 2), graph_gbs=round(B_alg / t_graph / 1e6, 1),
+                            sequential_gbs=round(B_alg / t_seq / 1e6, 1), bytes_alg=B_alg,
+                            rel_diff_batch_vs_sequential=err,
+                            schedule="PB (block-diagonal batch)" if k == 1 else "ROW (block-diagonal batch)",
+                            sequential="plain FP32 drop-in per matrix" if k == 1 else "ROW per matrix")
+        log(f"[batched] k={k:2d} graph {t_graph:.3f} ms  eager {t_eager:.3f} ms  sequential {t_seq:.3f} ms  "
+            f"({res[f'k{k}']['graph_gbs']} GB/s, x{res[f'k{k}']['speedup_vs_sequential']})  diff {err:.1e}")
+        del g
+    return res
+
+
 def cpu_baseline(M, m, n, nnz, budget_s=12.0):
     """The oracle's FastLoad_cpu (serial port of src/FastLoad_CPU.h:19-31) over the same 32-wide
     layout in FP32, repeated until ~budget_s of CPU time; 1 core."""
@@ -174,6 +272,7 @@ def main():
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
     ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_bf16x2,pb_bf16,pb_f64")
     ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_bf16x2,win_bf16,win_f64,pb_f32")
     args = ap.parse_args()
@@ -227,6 +326,8 @@ def main():
         out["stencil"] = {"workload": "27-point stencil on 128^3, |v| U[0.5,1.5] random sign, seed 128", "nnz": snnz,
                           "variants": sres}
         del smats
+    if args.batch > 0:
+        out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mats[32], m, n, nnz)
     print(json.dumps(out), flush=True)

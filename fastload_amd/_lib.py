@@ -61,6 +61,15 @@ SIGNATURES = {
     "fl_matrix_prepare_values": (C.c_int, [vp, C.c_int, vp]),
     "fl_spmv": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
     "fl_matrix_prepare_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "fl_matrix_prepare_rows": (C.c_int, [vp, vp]),
+    "fl_spmm": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
+    "fl_batch_assemble": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, i32p, i32p, C.POINTER(i32p), C.POINTER(i32p),
+                                    C.POINTER(f64p), vp, vp]),
+    "fl_graph_begin_capture": (C.c_int, [vp]),
+    "fl_graph_end_capture": (C.c_int, [vp, C.POINTER(vp)]),
+    "fl_graph_launch": (C.c_int, [vp, vp]),
+    "fl_graph_destroy": (C.c_int, [vp]),
+    "fl_zero_device": (C.c_int, [vp, C.c_size_t, vp]),
     "fl_gen_matrix": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int, vp, C.c_int, C.POINTER(i32p), C.POINTER(i32p),
                                 C.POINTER(f64p), i32p]),
     "fl_gen_vector": (C.c_int, [C.c_int, C.c_uint64, C.c_int, vp]),

@@ -191,6 +191,15 @@ class FastLoadMatrix:
         check(lib.fl_matrix_prepare_schedule(self.handle, kernel, batch_tiles, band_rows,
                                              stream.ptr if stream is not None else None), "fl_matrix_prepare_schedule")
 
+    def prepare_rows(self, stream=None):
+        check(lib.fl_matrix_prepare_rows(self.handle, stream.ptr if stream is not None else None),
+              "fl_matrix_prepare_rows")
+
+    def spmm(self, dX: DeviceArray, dY: DeviceArray, k: int, val_kind=VAL_F32, accumulate=False, stream=None):
+        """Y = A X (or Y += A X); X is n x k and Y is m x k, row-major, original column order."""
+        check(lib.fl_spmm(self.handle, val_kind, k, dX.ptr, dY.ptr, 1 if accumulate else 0,
+                          stream.ptr if stream is not None else None), "fl_spmm")
+
     def permute_x(self, dx: DeviceArray, dsx: DeviceArray, stream=None):
         fn = lib.fl_permute_x_f64 if dx.dtype == np.float64 else lib.fl_permute_x_f32
         check(fn(self.handle, dx.ptr, dsx.ptr, stream.ptr if stream is not None else None), "fl_permute_x")
@@ -210,6 +219,66 @@ class FastLoadMatrix:
             self.close()
         except Exception:
             pass
+
+
+# --------------------------------------------------------------------- batches, graphs
+def batch_assemble(mats):
+    """Block-diagonal assembly of [(m, n, colptr, rowidx, val), ...] ->
+    (M, N, colptr, rowidx, val, row_off, col_off) (include/fastload.h fl_batch_assemble)."""
+    count = len(mats)
+    keep = [(np.ascontiguousarray(c, np.int32), np.ascontiguousarray(r, np.int32), np.ascontiguousarray(v, np.float64))
+            for (_, _, c, r, v) in mats]
+    ms = np.array([m for (m, *_rest) in mats], np.int32)
+    ns = np.array([n for (_, n, *_rest) in mats], np.int32)
+    P = C.c_void_p * max(count, 1)
+    cps = P(*[k[0].ctypes.data for k in keep])
+    ris = P(*[k[1].ctypes.data if k[1].size else k[0].ctypes.data for k in keep])
+    vvs = P(*[k[2].ctypes.data if k[2].size else k[0].ctypes.data for k in keep])
+    M, N = C.c_int(), C.c_int()
+    cp, ri, v = i32p(), i32p(), f64p()
+    ro = np.zeros(count + 1, np.int32)
+    co = np.zeros(count + 1, np.int32)
+    check(lib.fl_batch_assemble(count, ms.ctypes.data, ns.ctypes.data, cps, ris, vvs, C.byref(M), C.byref(N),
+                                C.byref(cp), C.byref(ri), C.byref(v), ro.ctypes.data, co.ctypes.data), "fl_batch_assemble")
+    out_cp = _copy_out(cp, N.value + 1, np.int32)
+    nnz = int(out_cp[-1]) if out_cp.size else 0
+    out = (M.value, N.value, out_cp, _copy_out(ri, nnz, np.int32), _copy_out(v, nnz, np.float64), ro, co)
+    for p in (cp, ri, v):
+        lib.fl_free(C.cast(p, C.c_void_p))
+    del keep
+    return out
+
+
+class Graph:
+    """A hipGraph captured from the kernels enqueued on `stream` inside `with Graph(stream):`."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.exec = None
+
+    def __enter__(self):
+        check(lib.fl_graph_begin_capture(self.stream.ptr), "fl_graph_begin_capture")
+        return self
+
+    def __exit__(self, *exc):
+        h = C.c_void_p()
+        check(lib.fl_graph_end_capture(self.stream.ptr, C.byref(h)), "fl_graph_end_capture")
+        self.exec = h.value
+        return False
+
+    def launch(self):
+        check(lib.fl_graph_launch(self.exec, self.stream.ptr), "fl_graph_launch")
+
+    def __del__(self):
+        try:
+            if self.exec:
+                lib.fl_graph_destroy(self.exec)
+        except Exception:
+            pass
+
+
+def zero_device(d: DeviceArray, stream=None):
+    check(lib.fl_zero_device(d.ptr, d.nbytes, stream.ptr if stream is not None else None), "fl_zero_device")
 
 
 # ------------------------------------------------------------------------- generators

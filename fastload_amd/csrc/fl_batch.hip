@@ -1,0 +1,340 @@
+// Multi-right-hand-side SpMM (ROW schedule), block-diagonal batching of many matrices, zero
+// fill and hipGraph capture. No reference counterpart: the reference multiplies one matrix by
+// one vector per call (src/FastLoad_GPU.h:275-409).
+//
+// ROW: Y (m x k, row-major) = A * X (n x k, row-major, original column order). With k >= 8 a
+// FastLoad-style column sweep would need k scattered y updates per non-zero (global atomics run
+// at ~20 G cache lines/s on MI355X); instead the entries are re-ordered by row once, a row is
+// owned by k lanes, X rows are gathered whole (k*4 bytes, a full 128-byte line at k = 32) and
+// each Y row is written once, without atomics.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "fl_matrix.h"
+
+namespace fl {
+
+static unsigned sgb(long long count) { return std::max(1u, std::min<unsigned>(grid_for(count), 65536u)); }
+
+#define GSTRIDE(i, count) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (count); i += (long long)gridDim.x * blockDim.x)
+
+// original column of every entry (tile order): tile t, lane l -> perm[tile_colidx[t] + l]
+__global__ void k_col_of(int T, int W, const int *__restrict__ tile_ptr, const int *__restrict__ tile_len,
+                         const int *__restrict__ tile_colidx, const int *__restrict__ perm, int *__restrict__ col_of) {
+    GSTRIDE(g, (long long)T * W) {
+        const long long t = g / W;
+        const int l = (int)(g - t * W);
+        if (l < tile_len[t]) col_of[tile_ptr[t] + l] = perm[tile_colidx[t] + l];
+    }
+}
+
+__global__ void k_rows_key(const int *__restrict__ col, uint32_t *__restrict__ key, int *__restrict__ order, int count) {
+    GSTRIDE(i, count) {
+        key[i] = (uint32_t)col[i];
+        order[i] = (int)i;
+    }
+}
+
+__global__ void k_rows_key2(const int *__restrict__ row, const int *__restrict__ order, uint32_t *__restrict__ key,
+                            int count) {
+    GSTRIDE(i, count) key[i] = (uint32_t)row[order[i]];
+}
+
+__global__ void k_rows_fill(const int *__restrict__ order, const int *__restrict__ col_of,
+                            const double *__restrict__ val, int *__restrict__ rcol, double *__restrict__ rv64,
+                            float *__restrict__ rv32, int count) {
+    GSTRIDE(i, count) {
+        const int g = order[i];
+        rcol[i] = col_of[g];
+        const double v = val[g];
+        rv64[i] = v;
+        rv32[i] = (float)v;
+    }
+}
+
+__global__ void k_row_bounds(const uint32_t *__restrict__ k, int count, int m, int *__restrict__ start) {
+    GSTRIDE(r, (long long)m + 1) {
+        int lo = 0, hi = count;
+        while (lo < hi) {
+            const int mid = (int)(((long long)lo + hi) >> 1);
+            if (k[mid] < (uint32_t)r) lo = mid + 1; else hi = mid;
+        }
+        start[r] = lo;
+    }
+}
+
+void row_prepare(fl_matrix *A, hipStream_t s) {
+    if (A->row_ready) return;
+    const int nnz = A->nnz, m = A->m, T = A->tilenum;
+    A->row_ptr.alloc((size_t)m + 1);
+    A->row_col.alloc(std::max(nnz, 1));
+    A->row_val32.alloc(std::max(nnz, 1));
+    A->row_val64.alloc(std::max(nnz, 1));
+    if (nnz == 0) {
+        FL_HIP(hipMemsetAsync(A->row_ptr.p, 0, sizeof(int) * ((size_t)m + 1), s));
+        FL_HIP(hipStreamSynchronize(s));
+        A->row_ready = true;
+        return;
+    }
+    DevBuf<int> col_of(nnz), order(nnz);
+    DevBuf<uint32_t> key(nnz);
+    k_col_of<<<sgb((long long)T * A->W), kBlock, 0, s>>>(T, A->W, A->tile_ptr.p, A->tile_len.p, A->tile_colidx.p,
+                                                         A->perm.p, col_of.p);
+    // (row, original column) order: stable by column, then stable by row. The order inside a row
+    // then depends only on A, not on the column sort, so a block of a batch sums exactly as the
+    // same matrix alone.
+    k_rows_key<<<sgb(nnz), kBlock, 0, s>>>(col_of.p, key.p, order.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(A->n - 1, 0)), s);
+    k_rows_key2<<<sgb(nnz), kBlock, 0, s>>>(A->sortrowidx.p, order.p, key.p, nnz);
+    FL_LAUNCH_CHECK();
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(m - 1, 0)), s);
+    k_rows_fill<<<sgb(nnz), kBlock, 0, s>>>(order.p, col_of.p, A->val64.p, A->row_col.p, A->row_val64.p,
+                                            A->row_val32.p, nnz);
+    k_row_bounds<<<sgb((long long)m + 1), kBlock, 0, s>>>(key.p, nnz, m, A->row_ptr.p);
+    FL_LAUNCH_CHECK();
+    FL_HIP(hipStreamSynchronize(s));
+    A->row_ready = true;
+}
+
+// K lanes own one row (64/K rows per wave); lane j accumulates column j of the row's Y block.
+// Persistent: waves stride over row groups. ACC = accumulator type.
+template <typename T, typename ACC, int K, int UNR>
+__global__ __launch_bounds__(kBlock) void k_spmm_row(int m, const int *__restrict__ row_ptr,
+                                                     const int *__restrict__ row_col, const T *__restrict__ row_val,
+                                                     const T *__restrict__ X, T *__restrict__ Y, int accumulate) {
+    constexpr int RPW = 64 / K;  // rows per wave
+    const int lane = threadIdx.x & 63;
+    const int sub = lane / K, j = lane % K;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    const long long groups = ((long long)m + RPW - 1) / RPW;
+    for (long long gq = wave; gq < groups; gq += nwaves) {
+        const long long r = gq * RPW + sub;
+        if (r >= m) continue;
+        const int e0 = row_ptr[r], e1 = row_ptr[r + 1];
+        ACC acc = 0;
+        int e = e0;
+        for (; e + UNR <= e1; e += UNR) {
+            int c[UNR];
+            T v[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                c[u] = row_col[e + u];
+                v[u] = row_val[e + u];
+            }
+            T xv[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) xv[u] = X[(long long)c[u] * K + j];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) acc += (ACC)v[u] * (ACC)xv[u];
+        }
+        for (; e < e1; ++e) acc += (ACC)row_val[e] * (ACC)X[(long long)row_col[e] * K + j];
+        T *out = Y + r * K + j;
+        *out = accumulate ? (T)((ACC)*out + acc) : (T)acc;
+    }
+}
+
+template <typename T, typename ACC, int K>
+static void launch_row(const fl_matrix *A, const T *val, const T *X, T *Y, int accumulate, hipStream_t s) {
+    const long long groups = ((long long)A->m + (64 / K) - 1) / (64 / K);
+    const unsigned g = std::max(1u, std::min<unsigned>(grid_for(groups * 64), 256u * 8u));
+    k_spmm_row<T, ACC, K, 4><<<g, kBlock, 0, s>>>(A->m, A->row_ptr.p, A->row_col.p, val, X, Y, accumulate);
+    FL_LAUNCH_CHECK();
+}
+
+template <typename T, typename ACC>
+static int dispatch_row(const fl_matrix *A, const T *val, int k, const T *X, T *Y, int acc, hipStream_t s) {
+    switch (k) {
+        case 1: launch_row<T, ACC, 1>(A, val, X, Y, acc, s); return FL_OK;
+        case 2: launch_row<T, ACC, 2>(A, val, X, Y, acc, s); return FL_OK;
+        case 4: launch_row<T, ACC, 4>(A, val, X, Y, acc, s); return FL_OK;
+        case 8: launch_row<T, ACC, 8>(A, val, X, Y, acc, s); return FL_OK;
+        case 16: launch_row<T, ACC, 16>(A, val, X, Y, acc, s); return FL_OK;
+        case 32: launch_row<T, ACC, 32>(A, val, X, Y, acc, s); return FL_OK;
+        case 64: launch_row<T, ACC, 64>(A, val, X, Y, acc, s); return FL_OK;
+        default:
+            set_error("fl_spmm: k must be 1, 2, 4, 8, 16, 32 or 64");
+            return FL_ERR_INVALID;
+    }
+}
+
+int spmm_row(const fl_matrix *A, int kind, int k, const void *X, void *Y, int accumulate, hipStream_t s) {
+    if (!A->row_ready) {
+        set_error("fl_spmm: row order not prepared (fl_matrix_prepare_rows)");
+        return FL_ERR_INVALID;
+    }
+    if (A->m == 0) return FL_OK;
+    if (kind == FL_VAL_F64) return dispatch_row<double, double>(A, A->row_val64.p, k, (const double *)X, (double *)Y, accumulate, s);
+    if (kind == FL_VAL_F32) return dispatch_row<float, float>(A, A->row_val32.p, k, (const float *)X, (float *)Y, accumulate, s);
+    set_error("fl_spmm: FL_VAL_F64 or FL_VAL_F32 only");
+    return FL_ERR_INVALID;
+}
+
+// ------------------------------------------------------------------------- zero fill
+__global__ void k_zero(uint32_t *__restrict__ p, long long words) {
+    GSTRIDE(i, words) p[i] = 0u;
+}
+
+__global__ void k_zero_bytes(unsigned char *__restrict__ p, long long bytes) {
+    GSTRIDE(i, bytes) p[i] = 0;
+}
+
+// Zero fill with a kernel (graph-capturable; no memset node).
+void zero_fill(void *d, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    const long long words = (long long)(bytes / 4);
+    if (words) {
+        k_zero<<<sgb(words), kBlock, 0, s>>>((uint32_t *)d, words);
+        FL_LAUNCH_CHECK();
+    }
+    const long long tail = (long long)(bytes - (size_t)words * 4);
+    if (tail) {
+        k_zero_bytes<<<1, 64, 0, s>>>((unsigned char *)d + words * 4, tail);
+        FL_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace fl
+
+using namespace fl;
+
+extern "C" {
+
+int fl_matrix_prepare_rows(fl_matrix *A, void *stream) {
+    FL_API_BEGIN
+    if (!A) {
+        set_error("fl_matrix_prepare_rows: NULL");
+        return FL_ERR_INVALID;
+    }
+    row_prepare(A, (hipStream_t)stream);
+    return FL_OK;
+    FL_API_END
+}
+
+int fl_spmm(fl_matrix *A, int val_kind, int k, const void *d_X, void *d_Y, int accumulate, void *stream) {
+    FL_API_BEGIN
+    if (!A || (A->m && (!d_X || !d_Y))) {
+        set_error("fl_spmm: NULL");
+        return FL_ERR_INVALID;
+    }
+    return spmm_row(A, val_kind, k, d_X, d_Y, accumulate, (hipStream_t)stream);
+    FL_API_END
+}
+
+int fl_zero_device(void *d, size_t bytes, void *stream) {
+    FL_API_BEGIN
+    if (bytes && !d) {
+        set_error("fl_zero_device: NULL");
+        return FL_ERR_INVALID;
+    }
+    zero_fill(d, bytes, (hipStream_t)stream);
+    return FL_OK;
+    FL_API_END
+}
+
+int fl_batch_assemble(int count, const int *m, const int *n, const int *const *colptr, const int *const *rowidx,
+                      const double *const *val, int *M, int *N, int **colptr_out, int **rowidx_out, double **val_out,
+                      int *row_off, int *col_off) {
+    if (count < 0 || !m || !n || !colptr || !rowidx || !val || !M || !N || !colptr_out || !rowidx_out || !val_out ||
+        !row_off || !col_off) {
+        set_error("fl_batch_assemble: NULL or negative count");
+        return FL_ERR_INVALID;
+    }
+    long long MM = 0, NN = 0, NZ = 0;
+    row_off[0] = col_off[0] = 0;
+    for (int i = 0; i < count; ++i) {
+        if (m[i] < 0 || n[i] < 0 || !colptr[i]) {
+            set_error("fl_batch_assemble: matrix %d invalid", i);
+            return FL_ERR_INVALID;
+        }
+        MM += m[i];
+        NN += n[i];
+        NZ += colptr[i][n[i]];
+        if (MM > 0x7fffffffLL || NN > 0x7fffffffLL || NZ > 0x7fffffffLL) {
+            set_error("fl_batch_assemble: batch exceeds int32 indexing");
+            return FL_ERR_SIZE;
+        }
+        row_off[i + 1] = (int)MM;
+        col_off[i + 1] = (int)NN;
+    }
+    int *cp = (int *)malloc(sizeof(int) * ((size_t)NN + 1));
+    int *ri = (int *)malloc(sizeof(int) * (size_t)std::max<long long>(NZ, 1));
+    double *vv = (double *)malloc(sizeof(double) * (size_t)std::max<long long>(NZ, 1));
+    if (!cp || !ri || !vv) {
+        free(cp); free(ri); free(vv);
+        set_error("fl_batch_assemble: out of memory");
+        return FL_ERR_ALLOC;
+    }
+    long long pos = 0;
+    for (int i = 0; i < count; ++i) {
+        const int ro = row_off[i], co = col_off[i];
+        for (int c = 0; c < n[i]; ++c) {
+            cp[co + c] = (int)pos;
+            for (int e = colptr[i][c]; e < colptr[i][c + 1]; ++e, ++pos) {
+                if (rowidx[i][e] < 0 || rowidx[i][e] >= m[i]) {
+                    free(cp); free(ri); free(vv);
+                    set_error("fl_batch_assemble: matrix %d row index out of range", i);
+                    return FL_ERR_RANGE;
+                }
+                ri[pos] = ro + rowidx[i][e];
+                vv[pos] = val[i][e];
+            }
+        }
+    }
+    cp[NN] = (int)pos;
+    *M = (int)MM;
+    *N = (int)NN;
+    *colptr_out = cp;
+    *rowidx_out = ri;
+    *val_out = vv;
+    return FL_OK;
+}
+
+// ---------------------------------------------------------------------------- hipGraph
+int fl_graph_begin_capture(void *stream) {
+    FL_API_BEGIN
+    if (!stream) {
+        set_error("fl_graph_begin_capture: capture needs a created stream, not the default stream");
+        return FL_ERR_INVALID;
+    }
+    FL_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+    return FL_OK;
+    FL_API_END
+}
+
+int fl_graph_end_capture(void *stream, void **graph_exec) {
+    FL_API_BEGIN
+    if (!stream || !graph_exec) {
+        set_error("fl_graph_end_capture: NULL");
+        return FL_ERR_INVALID;
+    }
+    hipGraph_t g = nullptr;
+    FL_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    FL_HIP(e);
+    *graph_exec = ge;
+    return FL_OK;
+    FL_API_END
+}
+
+int fl_graph_launch(void *graph_exec, void *stream) {
+    FL_API_BEGIN
+    FL_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return FL_OK;
+    FL_API_END
+}
+
+int fl_graph_destroy(void *graph_exec) {
+    FL_API_BEGIN
+    if (graph_exec) FL_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return FL_OK;
+    FL_API_END
+}
+
+}  // extern "C"

@@ -49,6 +49,13 @@ struct fl_matrix {
     fl::DevBuf<int> pb_band_start, pb_run_off, pb_run_lstart, pb_run_dest, pb_run_len;
     fl::DevBuf<float> pb_bins32;
     fl::DevBuf<double> pb_bins64;
+
+    // ROW (multi-right-hand-side): the entries re-ordered by row (CSR of A, columns in the
+    // original numbering, so X needs no permutation): row_ptr [m+1], row_col/row_val [nnz].
+    bool row_ready = false;
+    fl::DevBuf<int> row_ptr, row_col;
+    fl::DevBuf<float> row_val32;
+    fl::DevBuf<double> row_val64;
 };
 
 namespace fl {

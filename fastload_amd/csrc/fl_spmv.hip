@@ -245,6 +245,7 @@ static void launch_fast(const fl_matrix *A, const V *val, const V *val_lo, const
 }
 
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, hipStream_t s);
+void zero_fill(void *d, size_t bytes, hipStream_t s);
 
 int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, hipStream_t s) {
     if (A->tilenum == 0) return FL_OK;
@@ -285,7 +286,7 @@ int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work,
             if (!A->bf16_hi.p) { set_error("fl_spmv: BF16 values not prepared"); return FL_ERR_INVALID; }
             if (!work) { set_error("fl_spmv: FL_VAL_BF16X2 needs a work buffer of 2*m floats"); return FL_ERR_INVALID; }
             float *yh = (float *)work, *yl = yh + A->m;
-            FL_HIP(hipMemsetAsync(yh, 0, sizeof(float) * 2 * (size_t)A->m, s));
+            zero_fill(yh, sizeof(float) * 2 * (size_t)A->m, s);
             launch_fast<uint16_t, float, float, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, yh, yl, s);
             k_recombine<<<sgrid(A->m), kBlock, 0, s>>>(yh, yl, (double *)y, A->m);
             FL_LAUNCH_CHECK();

@@ -207,6 +207,35 @@ int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d
  * first use. */
 int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int band_rows, void *stream);
 
+/* ------------------------------------ multiple right-hand sides (no reference counterpart)
+ * Y (m x k, row-major) = A * X  (accumulate = 0)  or  Y += A * X  (accumulate = 1), where X is
+ * n x k, row-major, in the ORIGINAL column order (no permutation). k in {1,2,4,8,16,32,64};
+ * val_kind FL_VAL_F64 (double X/Y, FP64 sums) or FL_VAL_F32 (float X/Y, FP32 sums).
+ * The entries are re-ordered by row once (fl_matrix_prepare_rows); k lanes own a row, X rows
+ * are gathered whole and each Y row is written once, without atomics. */
+int fl_matrix_prepare_rows(fl_matrix *A, void *stream);
+int fl_spmm(fl_matrix *A, int val_kind, int k, const void *d_X, void *d_Y, int accumulate, void *stream);
+
+/* --------------------------------------------- batches of independent matrices
+ * Block-diagonal assembly A = diag(A_0, ..., A_{count-1}) of host CSC matrices: one SpMV/SpMM
+ * with A computes every y_i = A_i x_i, so a whole batch runs through the same preprocessing and
+ * kernels in one launch per phase. Outputs are malloc'd (fl_free); row_off/col_off have
+ * count+1 entries (y_i = y[row_off[i] .. row_off[i+1]), x_i likewise). */
+int fl_batch_assemble(int count, const int *m, const int *n, const int *const *colptr, const int *const *rowidx,
+                      const double *const *val, int *M, int *N, int **colptr_out, int **rowidx_out,
+                      double **val_out, int *row_off, int *col_off);
+
+/* --------------------------------------------------------- hipGraph capture (plumbing)
+ * Capture the kernels enqueued on a created stream between begin and end into an executable
+ * graph; replay it with fl_graph_launch. The SpMV/SpMM entry points enqueue no memset nodes.
+ * Prepare values/schedules and run once eagerly before capturing (first calls allocate). */
+int fl_graph_begin_capture(void *stream);
+int fl_graph_end_capture(void *stream, void **graph_exec);
+int fl_graph_launch(void *graph_exec, void *stream);
+int fl_graph_destroy(void *graph_exec);
+/* Zero `bytes` of device memory with a kernel (capturable, unlike a memset node). */
+int fl_zero_device(void *d, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

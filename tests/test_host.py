@@ -186,3 +186,18 @@ def test_tile_len_closed_form_w64(seed):
     nnz = int(s.sum())
     t = ob.transform(n, nnz, s, np.zeros(nnz, np.int32), np.zeros(nnz), 64)
     assert np.array_equal(t["tile_len"], np.minimum(64, n - t["tile_colidx"]))
+
+
+def test_batch_assemble_block_diagonal():
+    import scipy.sparse as sp
+    mats = [fl.gen_matrix(fl.GEN_ZIPF, i, 30 + 17 * i, 20 + 11 * i, (2.0, 50)) for i in range(6)]
+    mats.append((4, 3, np.zeros(4, np.int32), np.zeros(0, np.int32), np.zeros(0)))
+    M, N, cp, ri, v, ro, co = fl.batch_assemble(mats)
+    B = sp.block_diag([sp.csc_matrix((m[4], m[3], m[2]), shape=(m[0], m[1])) for m in mats]).tocsc()
+    B.sort_indices()
+    assert (M, N) == B.shape and np.array_equal(cp, B.indptr) and np.array_equal(ri, B.indices)
+    assert np.array_equal(v, B.data)
+    assert ro.tolist() == list(np.cumsum([0] + [m[0] for m in mats]))
+    assert co.tolist() == list(np.cumsum([0] + [m[1] for m in mats]))
+    with pytest.raises(fl.FastLoadError):  # row index outside its block
+        fl.batch_assemble([(2, 1, np.array([0, 1], np.int32), np.array([5], np.int32), np.ones(1))])
