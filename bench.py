@@ -10,11 +10,14 @@ Variants (schedules x value storage):
   fast_*   64-wide tiles, level-table geometry, one global atomic per segment
   win_*    batches of 256 tiles, rows compacted to LDS slots, FP64 LDS sums, one atomic per row
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
+  row_*    entries re-ordered by row, lanes own rows, no atomics
+  learned_* the schedule the learned selector picks for the matrix (features and MLP inference
+           on the GPU, from the committed trained weights), in each value storage it supports
   *_f32 / *_f64 / *_bf16 / *_bf16x2: value storage (split hi+lo BF16 recombined in FP64)
 `value` = GB/s-equivalent (algorithmic bytes with FP32 values / step time) of the fastest
-variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported beside it.
-The stencil configuration (128^3 27-point, 55.7M nnz) is timed too and reported under
-"stencil" with the error of every value storage.
+learned_* variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported
+beside it. The stencil configuration (128^3 27-point, 55.7M nnz) is timed too and reported
+under "stencil" with the error of every value storage.
 
 Algorithmic bytes per SpMV: B = nnz*(s_val + 4) + n*s_x + m*s_y.
 """
@@ -41,7 +44,7 @@ def b_alg(nnz, n, m, s_val, s_x, s_y):
 
 
 VARIANTS = {
-    # name: (W, kernel, kind, s_val, batch_tiles, band_rows)
+    # name: (W, kernel, kind, s_val, batch_tiles, band_rows); kernel -1 = the row-ordered kernel
     "plain_f32": (32, 0, 1, 4, 0, 0),
     "plain_f64": (32, 0, 0, 8, 0, 0),
     "fast_f32": (64, 1, 1, 4, 0, 0),
@@ -53,6 +56,7 @@ VARIANTS = {
     "pb_bf16x2": (64, 3, 3, 4, 256, 16384),
     "pb_bf16": (64, 3, 2, 2, 256, 16384),
     "pb_f64": (64, 3, 0, 8, 256, 16384),
+    "row_f32": (64, -1, 1, 4, 0, 0),
     # schedule-parameter sweep points
     "win128_f32": (64, 2, 1, 4, 128, 0),
     "win64_f32": (64, 2, 1, 4, 64, 0),
@@ -61,23 +65,36 @@ VARIANTS = {
     "win128_f64": (64, 2, 0, 8, 128, 0),
     "pb128_f32": (64, 3, 1, 4, 128, 16384),
     "pb8k_f32": (64, 3, 1, 4, 256, 8192),
+    "pb4k_f32": (64, 3, 1, 4, 256, 4096),
     "pb128_8k_f32": (64, 3, 1, 4, 128, 8192),
 }
+KIND_SVAL = {0: ("f64", 8), 1: ("f32", 4), 2: ("bf16", 2), 3: ("bf16x2", 4)}
 
 
-def time_variant(fl, M, kernel, kind, dsx, dy, steps, warmup):
+def learned_variants(sched):
+    """Value-storage variants of the schedule the selector chose: FP32, split hi+lo BF16 and BF16
+    (the kinds that schedule supports without a work buffer)."""
+    kinds = [1, 3, 2] if sched.kernel in (2, 3) else ([1, 2] if sched.kernel >= 0 else [1])
+    return {f"learned_{KIND_SVAL[k][0]}": (sched.width, sched.kernel, k, KIND_SVAL[k][1], sched.batch_tiles,
+                                           sched.band_rows) for k in kinds}
+
+
+def time_variant(fl, run, dy, steps, warmup, zero=True):
+    """Step = (y = 0) + run(); events bracket every run() (the SpMV launches) and the whole loop."""
     from fastload_amd import Event
     for _ in range(warmup):
-        dy.zero()
-        M.spmv(dsx, dy, kernel, kind)
+        if zero:
+            fl.zero_device(dy)
+        run()
     fl.synchronize()
     ev = [(Event(), Event()) for _ in range(steps)]
     t0, t1 = Event(), Event()
     t0.record()
     for a, b in ev:
-        dy.zero()
+        if zero:
+            fl.zero_device(dy)
         a.record()
-        M.spmv(dsx, dy, kernel, kind)
+        run()
         b.record()
     t1.record()
     fl.synchronize()
@@ -86,17 +103,28 @@ def time_variant(fl, M, kernel, kind, dsx, dy, steps, warmup):
     return step_ms, float(kern.mean()), float(np.median(kern))
 
 
-def run_workload(fl, name, m, n, cp, ri, v, x, variants, steps, warmup):
+def select_schedule(fl, m, n, dcp, dri):
+    """The learned selector on the resident matrix: features on the GPU, MLP inference on the GPU."""
+    from fastload_amd import selector as sel
+    s = sel.Selector()
+    fl.synchronize()
+    t0 = time.perf_counter()
+    c, p, f = s.choose(m, n, dcp, dri)
+    ms = (time.perf_counter() - t0) * 1e3
+    return sel.SCHEDULES[c], dict(cls=c, schedule=sel.SCHEDULES[c].name, prob=round(p, 4), select_ms=round(ms, 3),
+                                  trained=s.mlp is not None)
+
+
+def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
+    """Times `variants` ({name: (W, kernel, kind, s_val, bt, R)}) on the resident CSC matrix."""
     from fastload_amd import DeviceArray
-    nnz = int(cp[n])
-    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
+    nnz = dri.count
     mats = {}
-    for W in sorted({VARIANTS[k][0] for k in variants} | {32}):
+    for W in sorted({t[0] for t in variants.values()} | {32}):
         t0 = time.time()
         mats[W] = fl.FastLoadMatrix(m, n, dcp, dri, dv, width=W)
         fl.synchronize()
         log(f"[{name}] preprocessed W={W}: levels={mats[W].levels} tiles={mats[W].tilenum} in {time.time() - t0:.2f}s")
-    del dcp, dri, dv
     dx64 = DeviceArray.from_host(x)
     dx32 = DeviceArray.from_host(x.astype(np.float32))
     sx = {}
@@ -113,21 +141,28 @@ def run_workload(fl, name, m, n, cp, ri, v, x, variants, steps, warmup):
     y_ref = y64.to_host()
     ref_norm = float(np.linalg.norm(y_ref))
     res = {}
-    for vname in variants:
-        W, kernel, kind, s_val, bt, R = VARIANTS[vname]
+    for vname, (W, kernel, kind, s_val, bt, R) in variants.items():
         M = mats[W]
-        if kind != fl.VAL_F64:
-            M.prepare(kind)
-        if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
-            t0 = time.time()
-            M.prepare_schedule(kernel, bt, max(R, 1024))
-            fl.synchronize()
-            prep_s = time.time() - t0
-        else:
-            prep_s = 0.0
+        t0 = time.time()
         s_x = 8 if kind == fl.VAL_F64 else 4
-        ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
-        step_ms, kern_ms, kern_med = time_variant(fl, M, kernel, kind, sx[(W, s_x)], ydev, steps, warmup)
+        if kernel < 0:  # row-ordered kernel: x in the original order, y written (not accumulated)
+            M.prepare_rows()
+            ydev = y32
+
+            def run(M=M):
+                M.spmm(dx32, y32, 1, fl.VAL_F32)
+        else:
+            if kind != fl.VAL_F64:
+                M.prepare(kind)
+            if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+                M.prepare_schedule(kernel, bt, max(R, 1024))
+            ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
+
+            def run(M=M, kernel=kernel, kind=kind, dsx=sx[(W, s_x)], ydev=ydev):
+                M.spmv(dsx, ydev, kernel, kind)
+        fl.synchronize()
+        prep_s = time.time() - t0
+        step_ms, kern_ms, kern_med = time_variant(fl, run, ydev, steps, warmup, zero=kernel >= 0)
         y = ydev.to_host().astype(np.float64)
         err = float(np.linalg.norm(y - y_ref) / ref_norm) if ref_norm else 0.0
         s_y = 8 if ydev is y64 else 4
@@ -138,7 +173,7 @@ def run_workload(fl, name, m, n, cp, ri, v, x, variants, steps, warmup):
                           gbs_equiv_f32=round(B32 / step_ms / 1e6, 2), kernel_gbs_equiv_f32=round(B32 / kern_ms / 1e6, 2),
                           bytes_alg=B, rel_err_l2=err, gflops=round(2 * nnz / step_ms / 1e6, 2),
                           schedule_prep_s=round(prep_s, 3))
-        log(f"[{name}] {vname:11s} step {step_ms:.4f} ms kernel {kern_ms:.4f} ms  {res[vname]['gbs']:7.1f} GB/s "
+        log(f"[{name}] {vname:14s} step {step_ms:.4f} ms kernel {kern_ms:.4f} ms  {res[vname]['gbs']:7.1f} GB/s "
             f"(f32-equiv {res[vname]['gbs_equiv_f32']:7.1f})  err {err:.2e}")
     return nnz, mats, res
 
@@ -146,7 +181,8 @@ def run_workload(fl, name, m, n, cp, ri, v, x, variants, steps, warmup):
 def run_batched(fl, count, seed, steps, warmup):
     """Batched configuration: `count` square CSC matrices, n log-uniform in [1e3, 2e5], column nnz
     Zipf(2.0) truncated at n, FP32, k = 1, 8, 32 right-hand sides. The batch runs as one
-    block-diagonal matrix (k = 1: PB schedule; k > 1: ROW kernel) captured in a hipGraph, against
+    block-diagonal matrix (k = 1: the schedule the learned selector picks for it; k > 1: the
+    row-ordered multi-RHS kernel) captured in a hipGraph, against
     `count` sequential per-matrix launches (plain drop-in FP32 kernel for k = 1, per-matrix ROW
     kernel for k > 1)."""
     from fastload_amd import DeviceArray, Event, Graph, Stream
@@ -159,11 +195,18 @@ def run_batched(fl, count, seed, steps, warmup):
     log(f"[batched] {count} matrices, rows {sizes.min()}..{sizes.max()}, sum rows {M_}, nnz {nnz} "
         f"generated+assembled in {time.time() - t0:.1f}s")
     s = Stream()
+    dcp, dri = DeviceArray.from_host(cp), DeviceArray.from_host(ri)
+    sched, chosen = select_schedule(fl, M_, N_, dcp, dri)
+    del dcp, dri
+    log(f"[batched] selector chose {chosen} for k = 1")
     B = fl.FastLoadMatrix(M_, N_, cp, ri, v, width=64)
-    B.prepare(fl.VAL_F32)
-    B.prepare_schedule(fl.KERNEL_PB, 256, 16384)
     B.prepare_rows()
-    res = {"matrices": count, "rows": int(M_), "nnz": nnz}
+    B1 = B if sched.width == 64 else fl.FastLoadMatrix(M_, N_, cp, ri, v, width=sched.width)
+    if sched.kernel >= 0:
+        B1.prepare(fl.VAL_F32)
+    if sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+        B1.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
+    res = {"matrices": count, "rows": int(M_), "nnz": nnz, "learned_schedule_k1": chosen}
 
     def timed(fn, reps):
         for _ in range(warmup):
@@ -191,13 +234,13 @@ def run_batched(fl, count, seed, steps, warmup):
         dX = DeviceArray.from_host(X.ravel())
         dY = DeviceArray(M_ * k, np.float32)
         B_alg = nnz * 8 + N_ * k * 4 + M_ * k * 4
-        if k == 1:
+        if k == 1 and sched.kernel >= 0:
             dsx = DeviceArray(N_, np.float32)
-            B.permute_x(dX, dsx)
+            B1.permute_x(dX, dsx)
 
             def batch_step():
                 fl.zero_device(dY, s)
-                B.spmv(dsx, dY, fl.KERNEL_PB, fl.VAL_F32, stream=s)
+                B1.spmv(dsx, dY, sched.kernel, fl.VAL_F32, stream=s)
         else:
             def batch_step():
                 B.spmm(dX, dY, k, fl.VAL_F32, stream=s)
@@ -233,7 +276,7 @@ def run_batched(fl, count, seed, steps, warmup):
                             speedup_vs_sequential=round(t_seq / t_graph, 2), graph_gbs=round(B_alg / t_graph / 1e6, 1),
                             sequential_gbs=round(B_alg / t_seq / 1e6, 1), bytes_alg=B_alg,
                             rel_diff_batch_vs_sequential=err,
-                            schedule="PB (block-diagonal batch)" if k == 1 else "ROW (block-diagonal batch)",
+                            schedule=(f"{sched.name} (learned) " if k == 1 else "row ") + "on the block-diagonal batch",
                             sequential="plain FP32 drop-in per matrix" if k == 1 else "ROW per matrix")
         log(f"[batched] k={k:2d} graph {t_graph:.3f} ms  eager {t_eager:.3f} ms  sequential {t_seq:.3f} ms  "
             f"({res[f'k{k}']['graph_gbs']} GB/s, x{res[f'k{k}']['speedup_vs_sequential']})  diff {err:.1e}")
@@ -273,25 +316,32 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
-    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_bf16x2,pb_bf16,pb_f64")
-    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_bf16x2,win_bf16,win_f64,pb_f32")
+    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
+    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,pb_f32,row_f32")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this FastLoad build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     import fastload_amd as fl
+    from fastload_amd import DeviceArray
 
     n = m = args.n
     t0 = time.time()
     _, _, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, args.seed, m, n, (1.8423, 50000))
     x = fl.gen_vector(fl.VEC_NORMAL, args.seed + 1, n)
     log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
-    nnz, mats, res = run_workload(fl, "powerlaw", m, n, cp, ri, v, x, args.variants.split(","), args.steps, args.warmup)
+    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
     del cp, ri, v
+    sched, chosen = select_schedule(fl, m, n, dcp, dri)
+    log(f"[powerlaw] selector chose {chosen}")
+    variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
+    variants.update(learned_variants(sched))
+    nnz, mats, res = run_workload(fl, "powerlaw", m, n, dcp, dri, dv, x, variants, args.steps, args.warmup)
+    del dcp, dri, dv
 
-    ok = {k: r for k, r in res.items() if r["rel_err_l2"] <= 1e-4 and not k.startswith("plain")}
-    best = max(ok, key=lambda k: ok[k]["gbs_equiv_f32"]) if ok else next(iter(res))
+    ok = {k: r for k, r in res.items() if r["rel_err_l2"] <= 1e-4 and k.startswith("learned")}
+    best = max(ok, key=lambda k: ok[k]["gbs_equiv_f32"])
     r = res[best]
     plain = res.get("plain_f32")
     kind_names = {"f32": "f32", "f64": "f64", "bf16": "bf16 values, f32 products, f64 accumulate",
@@ -310,26 +360,36 @@ def main():
         "dtype": kind_names[best.split("_", 1)[1]],
         "data": "synthetic",
         "config": {"workload": f"power-law CSC SpMV {m}x{n}, Zipf(1.8423, max 50000) column nnz, seed {args.seed}",
-                   "nnz": nnz, "variant": best, "rel_err_l2_vs_fp64": r["rel_err_l2"],
+                   "nnz": nnz, "variant": best, "learned_schedule": chosen, "rel_err_l2_vs_fp64": r["rel_err_l2"],
                    "plain_f32_gbs": plain["gbs"] if plain else None,
                    "speedup_vs_plain_f32": round(plain["step_ms"] / r["step_ms"], 3) if plain else None},
         "roofline": {"bound": "hbm", "achieved": r["kernel_gbs_equiv_f32"], "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4), "traffic": None},
         "variants": res,
     }
+    del mats
     if not args.no_stencil:
         _, _, scp, sri, sv = fl.gen_matrix(fl.GEN_STENCIL27, 128, 0, 0, (128,))
         sn = sm = 128 ** 3
         sx = fl.gen_vector(fl.VEC_NORMAL, 129, sn)
-        snnz, smats, sres = run_workload(fl, "stencil", sm, sn, scp, sri, sv, sx, args.stencil_variants.split(","),
-                                         args.steps, args.warmup)
+        dcp, dri, dv = DeviceArray.from_host(scp), DeviceArray.from_host(sri), DeviceArray.from_host(sv)
+        del scp, sri, sv
+        ssched, schosen = select_schedule(fl, sm, sn, dcp, dri)
+        log(f"[stencil] selector chose {schosen}")
+        svars = {k: VARIANTS[k] for k in args.stencil_variants.split(",") if k}
+        svars.update(learned_variants(ssched))
+        svars.update({k: VARIANTS[k] for k in ("win_bf16x2", "win_bf16")})
+        snnz, smats, sres = run_workload(fl, "stencil", sm, sn, dcp, dri, dv, sx, svars, args.steps, args.warmup)
         out["stencil"] = {"workload": "27-point stencil on 128^3, |v| U[0.5,1.5] random sign, seed 128", "nnz": snnz,
-                          "variants": sres}
-        del smats
+                          "learned_schedule": schosen, "variants": sres}
+        del smats, dcp, dri, dv
     if args.batch > 0:
         out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mats[32], m, n, nnz)
+        _, _, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, args.seed, m, n, (1.8423, 50000))
+        M32 = fl.FastLoadMatrix(m, n, cp, ri, v, width=32)
+        del cp, ri, v
+        out["cpu_baseline"] = cpu_baseline(M32, m, n, nnz)
     print(json.dumps(out), flush=True)
 
 

@@ -70,6 +70,16 @@ SIGNATURES = {
     "fl_graph_launch": (C.c_int, [vp, vp]),
     "fl_graph_destroy": (C.c_int, [vp]),
     "fl_zero_device": (C.c_int, [vp, C.c_size_t, vp]),
+    "fl_features": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
+    "fl_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
+    "fl_mlp_destroy": (C.c_int, [vp]),
+    "fl_mlp_param_count": (C.c_int, [vp]),
+    "fl_mlp_get_params": (C.c_int, [vp, vp]),
+    "fl_mlp_set_params": (C.c_int, [vp, vp]),
+    "fl_mlp_forward": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+    "fl_mlp_predict": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
+    "fl_mlp_gradients": (C.c_int, [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_float), vp]),
+    "fl_mlp_train_step": (C.c_int, [vp, vp, vp, C.c_int, C.c_float, C.POINTER(C.c_float), vp]),
     "fl_gen_matrix": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int, vp, C.c_int, C.POINTER(i32p), C.POINTER(i32p),
                                 C.POINTER(f64p), i32p]),
     "fl_gen_vector": (C.c_int, [C.c_int, C.c_uint64, C.c_int, vp]),

@@ -236,6 +236,41 @@ int fl_graph_destroy(void *graph_exec);
 /* Zero `bytes` of device memory with a kernel (capturable, unlike a memset node). */
 int fl_zero_device(void *d, size_t bytes, void *stream);
 
+/* ------------------------------------------ learned schedule selector (no reference counterpart)
+ * The reference runs one fixed schedule; its nearest hook is the unused column statistics of
+ * src/auxiliary.h:6-25 (mean, standard deviation, ratio of column lengths).
+ *
+ * fl_features: 40 features of a device CSC matrix, written to host `features`:
+ *   [0..31] fraction of columns whose length falls in bin b (b = 0: empty; b >= 1: length in
+ *           [2^(b-1), 2^b); bin 31 open-ended);
+ *   [32] log2(m+1)/24  [33] log2(n+1)/24  [34] log2(nnz+1)/32  [35] log2(mean row length+1)/8
+ *   [36] coefficient of variation of row lengths  [37] log2(max/mean row length + 1)/8
+ *   [38] fraction of empty rows  [39] mean over non-empty columns of (last row - first row)/m.
+ * Rows must be ascending inside each column (as fl_matrix_create requires). */
+int fl_features(int m, int n, int nnz, const int *d_colptr, const int *d_rowidx, float *features, void *stream);
+
+/* A 3-layer perceptron in -> h1 -> h2 -> out (ReLU hidden layers, logits out) whose matrix
+ * products run on MFMA with BF16 operands and FP32 accumulation; FP32 master weights, Adam.
+ * Parameters (fl_mlp_get_params / set_params / gradients order): W1 (in x h1, row-major), b1,
+ * W2, b2, W3, b3 -- fl_mlp_param_count floats. Inputs d_X (batch x in) and soft targets d_T
+ * (batch x out, rows summing to 1) are device FP32, row-major. Loss: mean over the batch of
+ * -sum_c T log softmax(logits). */
+typedef struct fl_mlp fl_mlp;
+int fl_mlp_create(fl_mlp **out, int n_in, int n_h1, int n_h2, int n_out, uint64_t seed);
+int fl_mlp_destroy(fl_mlp *P);
+int fl_mlp_param_count(const fl_mlp *P);
+int fl_mlp_get_params(const fl_mlp *P, float *host);
+int fl_mlp_set_params(fl_mlp *P, const float *host);
+/* logits (batch x out) to device d_logits */
+int fl_mlp_forward(fl_mlp *P, const float *d_X, int batch, float *d_logits, void *stream);
+/* argmax class per row to host classes[batch], its softmax probability to host prob (optional) */
+int fl_mlp_predict(fl_mlp *P, const float *d_X, int batch, int *classes, float *prob, void *stream);
+/* gradients of the loss at the current parameters to host (param_count floats); loss optional */
+int fl_mlp_gradients(fl_mlp *P, const float *d_X, const float *d_T, int batch, float *host_grads, float *loss,
+                     void *stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8, learning rate lr); loss before the step */
+int fl_mlp_train_step(fl_mlp *P, const float *d_X, const float *d_T, int batch, float lr, float *loss, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
