@@ -71,12 +71,25 @@ VARIANTS = {
 KIND_SVAL = {0: ("f64", 8), 1: ("f32", 4), 2: ("bf16", 2), 3: ("bf16x2", 4)}
 
 
-def learned_variants(sched):
+def learned_variants(sched, names=("f32", "bf16x2", "bf16")):
     """Value-storage variants of the schedule the selector chose: FP32, split hi+lo BF16 and BF16
-    (the kinds that schedule supports without a work buffer)."""
+    (those of `names` that the schedule supports without a work buffer)."""
     kinds = [1, 3, 2] if sched.kernel in (2, 3) else ([1, 2] if sched.kernel >= 0 else [1])
     return {f"learned_{KIND_SVAL[k][0]}": (sched.width, sched.kernel, k, KIND_SVAL[k][1], sched.batch_tiles,
-                                           sched.band_rows) for k in kinds}
+                                           sched.band_rows) for k in kinds if KIND_SVAL[k][0] in names}
+
+
+def profiled_traffic(schedule: str, kind: str, workload: str):
+    """HBM bytes per SpMV from a committed rocprofv3 counter summary of this workload, schedule
+    and value storage (profiles/traffic_*.md, line `traffic_bytes_per_spmv: <workload>
+    <schedule> <kind> <bytes>`), or None when no such profile exists."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_*.md"))):
+        for line in open(f):
+            parts = line.strip().strip("`").split()
+            if len(parts) == 5 and parts[0] == "traffic_bytes_per_spmv:" and parts[1:4] == [workload, schedule, kind]:
+                return int(parts[4])
+    return None
 
 
 def time_variant(fl, run, dy, steps, warmup, zero=True):
@@ -318,6 +331,8 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
     ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
     ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,pb_f32,row_f32")
+    ap.add_argument("--learned-kinds", default="f32,bf16x2,bf16",
+                    help="value storages timed for the learned schedule")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this FastLoad build runs on one GPU; --gpus must be 1"}))
@@ -336,7 +351,7 @@ def main():
     sched, chosen = select_schedule(fl, m, n, dcp, dri)
     log(f"[powerlaw] selector chose {chosen}")
     variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
-    variants.update(learned_variants(sched))
+    variants.update(learned_variants(sched, args.learned_kinds.split(",")))
     nnz, mats, res = run_workload(fl, "powerlaw", m, n, dcp, dri, dv, x, variants, args.steps, args.warmup)
     del dcp, dri, dv
 
@@ -364,7 +379,8 @@ def main():
                    "plain_f32_gbs": plain["gbs"] if plain else None,
                    "speedup_vs_plain_f32": round(plain["step_ms"] / r["step_ms"], 3) if plain else None},
         "roofline": {"bound": "hbm", "achieved": r["kernel_gbs_equiv_f32"], "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4), "traffic": None},
+                     "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4),
+                     "traffic": profiled_traffic(chosen["schedule"], best.split("_", 1)[1], "powerlaw")},
         "variants": res,
     }
     del mats
@@ -377,11 +393,13 @@ def main():
         ssched, schosen = select_schedule(fl, sm, sn, dcp, dri)
         log(f"[stencil] selector chose {schosen}")
         svars = {k: VARIANTS[k] for k in args.stencil_variants.split(",") if k}
-        svars.update(learned_variants(ssched))
+        svars.update(learned_variants(ssched, args.learned_kinds.split(",")))
         svars.update({k: VARIANTS[k] for k in ("win_bf16x2", "win_bf16")})
         snnz, smats, sres = run_workload(fl, "stencil", sm, sn, dcp, dri, dv, sx, svars, args.steps, args.warmup)
         out["stencil"] = {"workload": "27-point stencil on 128^3, |v| U[0.5,1.5] random sign, seed 128", "nnz": snnz,
-                          "learned_schedule": schosen, "variants": sres}
+                          "learned_schedule": schosen,
+                          "traffic_bytes_learned_f32": profiled_traffic(schosen["schedule"], "f32", "stencil"),
+                          "variants": sres}
         del smats, dcp, dri, dv
     if args.batch > 0:
         out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup)
