@@ -11,8 +11,9 @@ Variants (schedules x value storage):
   win_*    batches of 256 tiles, rows compacted to LDS slots, FP64 LDS sums, one atomic per row
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
   row_*    entries re-ordered by row, lanes own rows, no atomics
-  learned_* the schedule the learned selector picks for the matrix (features and MLP inference
-           on the GPU, from the committed trained weights), in each value storage it supports
+  learned_* the schedule the learned selector picks for the matrix: features and MLP inference
+           on the GPU (committed trained weights) rank the 12 schedules, the three best-ranked
+           are timed on the matrix and the fastest is used; in each value storage it supports
   *_f32 / *_f64 / *_bf16 / *_bf16x2: value storage (split hi+lo BF16 recombined in FP64)
 `value` = GB/s-equivalent (algorithmic bytes with FP32 values / step time) of the fastest
 learned_* variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported
@@ -116,16 +117,24 @@ def time_variant(fl, run, dy, steps, warmup, zero=True):
     return step_ms, float(kern.mean()), float(np.median(kern))
 
 
-def select_schedule(fl, m, n, dcp, dri):
-    """The learned selector on the resident matrix: features on the GPU, MLP inference on the GPU."""
+def select_schedule(fl, m, n, dcp, dri, dv, top=3):
+    """The learned selector on the resident matrix: features and MLP inference on the GPU rank the
+    12 schedules; the `top` best-ranked are timed on the matrix (model-guided tuning) and the
+    fastest is used. Both the network's first choice and the tuned one are reported."""
     from fastload_amd import selector as sel
     s = sel.Selector()
     fl.synchronize()
     t0 = time.perf_counter()
-    c, p, f = s.choose(m, n, dcp, dri)
+    c1, p, f = s.choose(m, n, dcp, dri)
+    ms1 = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    c, times, _ = s.choose_refined(m, n, dcp, dri, dv, top=top)
     ms = (time.perf_counter() - t0) * 1e3
-    return sel.SCHEDULES[c], dict(cls=c, schedule=sel.SCHEDULES[c].name, prob=round(p, 4), select_ms=round(ms, 3),
-                                  trained=s.mlp is not None)
+    return sel.SCHEDULES[c], dict(cls=c, schedule=sel.SCHEDULES[c].name, network_choice=sel.SCHEDULES[c1].name,
+                                  network_prob=round(p, 4), select_ms=round(ms1, 3), tuned_top=top,
+                                  tuned_ms={sel.SCHEDULES[k].name: (round(v, 4) if v is not None else None)
+                                            for k, v in times.items()},
+                                  tuning_s=round(ms / 1e3, 3), trained=s.mlp is not None)
 
 
 def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
@@ -208,9 +217,9 @@ def run_batched(fl, count, seed, steps, warmup):
     log(f"[batched] {count} matrices, rows {sizes.min()}..{sizes.max()}, sum rows {M_}, nnz {nnz} "
         f"generated+assembled in {time.time() - t0:.1f}s")
     s = Stream()
-    dcp, dri = DeviceArray.from_host(cp), DeviceArray.from_host(ri)
-    sched, chosen = select_schedule(fl, M_, N_, dcp, dri)
-    del dcp, dri
+    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
+    sched, chosen = select_schedule(fl, M_, N_, dcp, dri, dv)
+    del dcp, dri, dv
     log(f"[batched] selector chose {chosen} for k = 1")
     B = fl.FastLoadMatrix(M_, N_, cp, ri, v, width=64)
     B.prepare_rows()
@@ -382,7 +391,7 @@ def main():
     log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
     dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
     del cp, ri, v
-    sched, chosen = select_schedule(fl, m, n, dcp, dri)
+    sched, chosen = select_schedule(fl, m, n, dcp, dri, dv)
     log(f"[powerlaw] selector chose {chosen}")
     variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
     variants.update(learned_variants(sched, args.learned_kinds.split(",")))
@@ -424,7 +433,7 @@ def main():
         sx = fl.gen_vector(fl.VEC_NORMAL, 129, sn)
         dcp, dri, dv = DeviceArray.from_host(scp), DeviceArray.from_host(sri), DeviceArray.from_host(sv)
         del scp, sri, sv
-        ssched, schosen = select_schedule(fl, sm, sn, dcp, dri)
+        ssched, schosen = select_schedule(fl, sm, sn, dcp, dri, dv)
         log(f"[stencil] selector chose {schosen}")
         svars = {k: VARIANTS[k] for k in args.stencil_variants.split(",") if k}
         svars.update(learned_variants(ssched, args.learned_kinds.split(",")))

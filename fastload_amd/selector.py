@@ -243,10 +243,38 @@ class Selector:
             return DEFAULT_CLASS, p
         return c, p
 
+    def rank_features(self, f) -> list[int]:
+        """Schedules ordered by the network's preference (the default alone as fallback)."""
+        f = np.asarray(f, np.float32)
+        if self.mlp is None or not np.all(np.isfinite(f)) or f[34] <= 0.0:
+            return [DEFAULT_CLASS]
+        logits = self.mlp.forward(f[None, :])[0]
+        return [int(c) for c in np.argsort(-logits, kind="stable")]
+
     def choose(self, m, n, dcolptr, drowidx, stream=None) -> tuple[int, float, np.ndarray]:
         f = features(m, n, dcolptr, drowidx, stream)
         c, p = self.choose_features(f)
         return c, p, f
+
+    def choose_refined(self, m, n, dcolptr, drowidx, dval, top: int = 3, reps: int = 10):
+        """Model-guided tuning: the network ranks the schedules, the `top` best-ranked are timed
+        on this matrix (`reps` steps each, after a warm-up step) and the fastest is returned with
+        the measured times. One-time cost: preparing and timing `top` schedules."""
+        f = features(m, n, dcolptr, drowidx)
+        cands = self.rank_features(f)[:max(top, 1)]
+        if len(cands) == 1:
+            return cands[0], {cands[0]: None}, f
+        mats, times = {}, {}
+        dx = DeviceArray.from_host(np.ones(n, np.float32))
+        dy = DeviceArray(max(m, 1), np.float32)
+        for c in cands:
+            run = ScheduledSpMV(m, n, dcolptr, drowidx, dval, c, mats)
+            run.set_x(dx)
+            times[c] = time_schedule(run.step, dy, reps_min=reps, reps_max=reps, rounds=1)
+        for M in mats.values():
+            M.close()
+        best = min(times, key=times.get)
+        return best, times, f
 
 
 # ---------------------------------------------------------------- schedule execution
@@ -505,6 +533,11 @@ def main(argv=None):
     fixed = np.exp(np.log(T[tr] / T[tr].min(axis=1, keepdims=True)).mean(axis=0))
     bf = int(np.argmin(fixed))
     rep["best_fixed"] = dict(cls=bf, name=SCHEDULES[bf].name, **evaluate(np.full(len(te), bf), T[te]))
+    logits = mlp.forward(F[te])
+    for k in (2, 3):
+        topk = np.argsort(-logits, axis=1, kind="stable")[:, :k]
+        pick = topk[np.arange(len(te)), np.argmin(np.take_along_axis(T[te], topk, axis=1), axis=1)]
+        rep[f"learned_top{k}_timed"] = evaluate(pick, T[te])
     rep["chosen_hist"] = {SCHEDULES[c].name: int((cls == c).sum()) for c in range(N_CLASSES)}
     rep["best_hist"] = {SCHEDULES[c].name: int((T[te].argmin(1) == c).sum()) for c in range(N_CLASSES)}
     print(json.dumps(rep, indent=1))

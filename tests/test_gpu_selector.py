@@ -183,3 +183,23 @@ def test_selected_schedule_end_to_end(gpu):
     run.step(dy)
     yo, S = ob.spmv_oracle(m, n, cp, ri, v.astype(np.float32), x)
     assert rel_err(dy.to_host().astype(np.float64), yo, S) <= 1e-5
+
+
+def test_refined_choice_times_the_top_ranked(gpu):
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, 14, 100000, 100000, (1.9, 10000))
+    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
+    s = sel.Selector()
+    ranked = s.rank_features(sel.features(m, n, dcp, dri))
+    assert sorted(ranked) == list(range(sel.N_CLASSES))
+    c1, _, _ = s.choose(m, n, dcp, dri)
+    assert ranked[0] == c1
+    c, times, f = s.choose_refined(m, n, dcp, dri, dv, top=3)
+    assert set(times) == set(ranked[:3]) and c == min(times, key=times.get)
+    assert all(t > 0 for t in times.values())
+    run = sel.ScheduledSpMV(m, n, dcp, dri, dv, c)
+    x = np.random.default_rng(2).standard_normal(n).astype(np.float32)
+    run.set_x(DeviceArray.from_host(x))
+    dy = DeviceArray(m, np.float32)
+    run.step(dy)
+    yo, S = ob.spmv_oracle(m, n, cp, ri, v.astype(np.float32), x)
+    assert rel_err(dy.to_host().astype(np.float64), yo, S) <= 1e-5

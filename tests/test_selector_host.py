@@ -209,3 +209,8 @@ def test_trained_network_gains_over_default_and_fixed_schedule():
     assert learned["geomean_speedup_vs_default"] > 1.5
     assert learned["geomean_regret"] < fixed["geomean_regret"]
     assert learned["geomean_regret"] < 1.10
+    # model-guided tuning: timing the three best-ranked schedules can only lower the regret
+    top3 = np.argsort(-logits, axis=1, kind="stable")[:, :3]
+    pick = top3[np.arange(len(te)), np.argmin(np.take_along_axis(T[te], top3, axis=1), axis=1)]
+    tuned = sel.evaluate(pick, T[te])
+    assert tuned["geomean_regret"] <= learned["geomean_regret"] and tuned["exact_best"] >= learned["exact_best"]
