@@ -17,7 +17,7 @@ Schedule space (tile width x reduction strategy x batch/band parameters):
   11 row         entries re-ordered by row (CSR), lanes own rows, no atomics (fl_spmm, k=1)
 
 Training data come from exhaustive sweeps: every schedule is timed on every matrix of a seeded
-synthetic population (`c5_spec`), with the matrix and x resident on the GPU; one step is y = 0
+synthetic population (`population_spec`), with the matrix and x resident on the GPU; one step is y = 0
 followed by one SpMV in FP32 (value storage FP32, FP64 LDS sums where the schedule has them).
 Preprocessing time is not part of a step (it is paid once per matrix).
 """
@@ -321,7 +321,7 @@ class ScheduledSpMV:
 
 
 # ------------------------------------------------------------------ sweep population
-C5_FAMILIES = ("zipf", "banded", "blockdiag", "uniform", "zipf_dense")
+POPULATION_FAMILIES = ("zipf", "banded", "blockdiag", "uniform", "zipf_dense")
 NNZ_CAP = 72_000_000
 N_MIN, N_MAX = 10_000, 8_388_608
 
@@ -332,7 +332,7 @@ def _zipf_mean(a, kmax):
     return float((k * w).sum() / w.sum())
 
 
-def c5_spec(i: int, base_seed: int = 20240):
+def population_spec(i: int, base_seed: int = 20240):
     """Matrix i of the selector population: (family name, gen family, m, n, params, seed).
 
     Family i % 5; columns log-uniform in [10k, 8M], square; family parameters:
@@ -345,7 +345,7 @@ def c5_spec(i: int, base_seed: int = 20240):
     Columns (and rows with them) are then reduced, if needed, so the expected non-zero count stays
     below 72M, which keeps a sweep bounded while covering the 67M-entry power-law workload."""
     rng = np.random.default_rng([base_seed, i])
-    fam = C5_FAMILIES[i % 5]
+    fam = POPULATION_FAMILIES[i % 5]
     n = int(round(math.exp(rng.uniform(math.log(N_MIN), math.log(N_MAX)))))
     seed = base_seed * 100_000 + i
     if fam in ("zipf", "zipf_dense"):
@@ -373,7 +373,7 @@ def c5_spec(i: int, base_seed: int = 20240):
     return fam, GEN_UNIFORM, n, n, (d,), seed
 
 
-def c5_split(count: int = 2000):
+def population_split(count: int = 2000):
     """Held-out set: every fifth group of five consecutive matrices (400 of 2,000, 80 per
     family); the rest train."""
     idx = np.arange(count)
@@ -408,7 +408,7 @@ def time_schedule(run, dy, reps_min=5, reps_max=60, target_ms=3.0, rounds=3):
 def sweep_one(i: int, base_seed: int = 20240, rounds: int = 3):
     """Features, per-schedule step times (ms) and per-schedule relative L2 difference to the
     default schedule's y for matrix i of the population."""
-    fam, gfam, m, n, params, seed = c5_spec(i, base_seed)
+    fam, gfam, m, n, params, seed = population_spec(i, base_seed)
     t0 = time.time()
     m, n, cp, ri, v = gen_matrix(gfam, seed, m, n, params)
     nnz = int(cp[n])
@@ -505,7 +505,7 @@ def main(argv=None):
         sweep(a.start, a.stop, a.out)
         return 0
     ids, F, T, E, M = load_dataset(a.data)
-    tr_all, te_all = c5_split(2000)
+    tr_all, te_all = population_split(2000)
     tr = np.flatnonzero(np.isin(ids, tr_all))
     te = np.flatnonzero(np.isin(ids, te_all))
     rep = dict(train=len(tr), test=len(te))

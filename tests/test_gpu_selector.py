@@ -140,7 +140,7 @@ def test_selector_gain_on_held_out_matrices(gpu):
     the default schedule and the best fixed schedule; a network retrained here from the sweep
     with the GPU training kernels does too."""
     ids, F, T, E, M = sel.load_dataset([sel.DATASET_PATH])
-    tr, te = sel.c5_split(2000)
+    tr, te = sel.population_split(2000)
     s = sel.Selector()
     chosen = np.array([s.choose_features(F[i])[0] for i in te])
     learned = sel.evaluate(chosen, T[te])

@@ -123,10 +123,10 @@ def test_schedule_table():
 
 
 def test_population_and_split():
-    specs = [sel.c5_spec(i) for i in range(2000)]
-    assert specs[17] == sel.c5_spec(17)
+    specs = [sel.population_spec(i) for i in range(2000)]
+    assert specs[17] == sel.population_spec(17)
     fams = [s[0] for s in specs]
-    assert {f: fams.count(f) for f in set(fams)} == {f: 400 for f in sel.C5_FAMILIES}
+    assert {f: fams.count(f) for f in set(fams)} == {f: 400 for f in sel.POPULATION_FAMILIES}
     for fam, g, m, n, p, seed in specs:
         assert m == n and 1000 <= n <= sel.N_MAX
         if fam == "zipf":
@@ -142,9 +142,9 @@ def test_population_and_split():
             exp_nnz = n * n * p[0]
             assert 1e-6 <= p[0] <= 1e-3
         assert exp_nnz <= sel.NNZ_CAP * 1.001
-    tr, te = sel.c5_split(2000)
+    tr, te = sel.population_split(2000)
     assert len(tr) == 1600 and len(te) == 400 and not set(tr) & set(te)
-    assert all(sum(1 for i in te if fams[i] == f) == 80 for f in sel.C5_FAMILIES)
+    assert all(sum(1 for i in te if fams[i] == f) == 80 for f in sel.POPULATION_FAMILIES)
 
 
 def test_soft_targets_and_regret():
@@ -180,7 +180,7 @@ def test_dataset_is_complete_and_consistent():
     assert np.all(np.isfinite(T)) and np.all(T > 0)
     assert np.nanmax(E) <= 1e-4  # every schedule's FP32 result agrees with the default's
     for i in (0, 1, 3, 5, 8):  # small matrices: regenerate on the host and recompute the features
-        fam, g, m, n, p, seed = sel.c5_spec(i)
+        fam, g, m, n, p, seed = sel.population_spec(i)
         assert M[i]["family"] == fam and M[i]["n"] == n
         if M[i]["nnz"] > 3_000_000:
             continue
@@ -195,7 +195,7 @@ def test_trained_network_gains_over_default_and_fixed_schedule():
     GPU) on the 400 held-out matrices, must beat the default schedule and the best single fixed
     schedule chosen on the training matrices."""
     ids, F, T, E, M = sel.load_dataset([sel.DATASET_PATH])
-    tr, te = sel.c5_split(2000)
+    tr, te = sel.population_split(2000)
     with np.load(sel.WEIGHTS_PATH, allow_pickle=False) as z:
         params, dims = z["params"], tuple(int(d) for d in z["dims"])
     logits = so.mlp_forward(params, dims, F[te], rounding=True)[0]
