@@ -80,6 +80,8 @@ SIGNATURES = {
     "fl_mlp_predict": (C.c_int, [vp, vp, C.c_int, vp, vp, vp]),
     "fl_mlp_gradients": (C.c_int, [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_float), vp]),
     "fl_mlp_train_step": (C.c_int, [vp, vp, vp, C.c_int, C.c_float, C.POINTER(C.c_float), vp]),
+    "fl_schedule_info": (C.c_int, [C.c_int, i32p, i32p, i32p, i32p]),
+    "fl_select_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, i32p, C.POINTER(C.c_float), vp]),
     "fl_gen_matrix": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_int, vp, C.c_int, C.POINTER(i32p), C.POINTER(i32p),
                                 C.POINTER(f64p), i32p]),
     "fl_gen_vector": (C.c_int, [C.c_int, C.c_uint64, C.c_int, vp]),

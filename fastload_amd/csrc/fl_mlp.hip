@@ -349,6 +349,45 @@ int fl_features(int m, int n, int nnz, const int *d_colptr, const int *d_rowidx,
     FL_API_END
 }
 
+// Schedule space of the selector: tile width x reduction strategy x batch/band parameters.
+// Class 0 is the reference schedule (32-wide tiles, segmented sums, atomics per segment).
+static const int kSchedules[FL_SCHEDULES][4] = {
+    {32, FL_KERNEL_DROPIN, 0, 0},   {64, FL_KERNEL_DROPIN, 0, 0},  {64, FL_KERNEL_FAST, 0, 0},
+    {64, FL_KERNEL_WIN, 64, 0},     {64, FL_KERNEL_WIN, 128, 0},   {64, FL_KERNEL_WIN, 256, 0},
+    {64, FL_KERNEL_PB, 128, 4096},  {64, FL_KERNEL_PB, 128, 16384}, {64, FL_KERNEL_PB, 256, 4096},
+    {64, FL_KERNEL_PB, 256, 8192},  {64, FL_KERNEL_PB, 256, 16384}, {64, FL_KERNEL_ROW, 0, 0},
+};
+
+int fl_schedule_info(int cls, int *width, int *kernel, int *batch_tiles, int *band_rows) {
+    if (cls < 0 || cls >= FL_SCHEDULES) return mlp_invalid("fl_schedule_info: class out of range");
+    if (width) *width = kSchedules[cls][0];
+    if (kernel) *kernel = kSchedules[cls][1];
+    if (batch_tiles) *batch_tiles = kSchedules[cls][2];
+    if (band_rows) *band_rows = kSchedules[cls][3];
+    return FL_OK;
+}
+
+int fl_select_schedule(fl_mlp *P, int m, int n, int nnz, const int *d_colptr, const int *d_rowidx, int *cls,
+                       float *prob, void *stream) {
+    FL_API_BEGIN
+    if (!cls) return mlp_invalid("fl_select_schedule: NULL class");
+    *cls = 0;
+    if (prob) *prob = 0.f;
+    if (P && (P->d[0] != 40 || P->d[3] != FL_SCHEDULES))
+        return mlp_invalid("fl_select_schedule: network must map 40 features to the 12 schedules");
+    float f[40];
+    const int rc = fl_features(m, n, nnz, d_colptr, d_rowidx, f, stream);
+    if (rc != FL_OK) return rc;
+    if (!P || nnz == 0) return FL_OK;  // fallback: the reference schedule
+    for (float v : f)
+        if (!std::isfinite(v)) return FL_OK;
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf<float> df(40);
+    FL_HIP(hipMemcpyAsync(df.p, f, sizeof(f), hipMemcpyHostToDevice, s));
+    return fl_mlp_predict(P, df.p, 1, cls, prob, stream);
+    FL_API_END
+}
+
 int fl_mlp_create(fl_mlp **out, int n_in, int n_h1, int n_h2, int n_out, uint64_t seed) {
     FL_API_BEGIN
     if (!out || n_in <= 0 || n_h1 <= 0 || n_h2 <= 0 || n_out <= 0) return mlp_invalid("fl_mlp_create: bad dims");

@@ -51,20 +51,29 @@ class Schedule:
     band_rows: int = 0
 
 
-SCHEDULES = (
-    Schedule("dropin_w32", 32, KERNEL_DROPIN),
-    Schedule("dropin_w64", 64, KERNEL_DROPIN),
-    Schedule("fast_w64", 64, KERNEL_FAST),
-    Schedule("win64", 64, KERNEL_WIN, 64),
-    Schedule("win128", 64, KERNEL_WIN, 128),
-    Schedule("win256", 64, KERNEL_WIN, 256),
-    Schedule("pb128_r4k", 64, KERNEL_PB, 128, 4096),
-    Schedule("pb128_r16k", 64, KERNEL_PB, 128, 16384),
-    Schedule("pb256_r4k", 64, KERNEL_PB, 256, 4096),
-    Schedule("pb256_r8k", 64, KERNEL_PB, 256, 8192),
-    Schedule("pb256_r16k", 64, KERNEL_PB, 256, 16384),
-    Schedule("row", 64, KERNEL_ROW),
-)
+def _schedule_table():
+    """The 12 schedules, read from the library (fl_schedule_info) so C and Python callers share
+    one table; names follow the parameters (dropin_w32, win128, pb256_r16k, row, ...)."""
+    kname = {KERNEL_DROPIN: "dropin", KERNEL_FAST: "fast", KERNEL_WIN: "win", KERNEL_PB: "pb", KERNEL_ROW: "row"}
+    out = []
+    for c in range(FL_SCHEDULES):
+        w, k, bt, r = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib.fl_schedule_info(c, C.byref(w), C.byref(k), C.byref(bt), C.byref(r)), "fl_schedule_info")
+        w, k, bt, r = w.value, k.value, bt.value, r.value
+        if k in (KERNEL_DROPIN, KERNEL_FAST):
+            name = f"{kname[k]}_w{w}"
+        elif k == KERNEL_WIN:
+            name = f"win{bt}"
+        elif k == KERNEL_PB:
+            name = f"pb{bt}_r{r // 1024}k"
+        else:
+            name = "row"
+        out.append(Schedule(name, w, k, bt, r))
+    return tuple(out)
+
+
+FL_SCHEDULES = 12
+SCHEDULES = _schedule_table()
 N_CLASSES = len(SCHEDULES)
 DEFAULT_CLASS = 0
 
@@ -252,9 +261,17 @@ class Selector:
         return [int(c) for c in np.argsort(-logits, kind="stable")]
 
     def choose(self, m, n, dcolptr, drowidx, stream=None) -> tuple[int, float, np.ndarray]:
-        f = features(m, n, dcolptr, drowidx, stream)
-        c, p = self.choose_features(f)
-        return c, p, f
+        """The network's schedule for a device CSC matrix (fl_select_schedule: features and
+        inference on the GPU), its probability, and the features."""
+        nnz = drowidx.count if drowidx is not None else 0
+        cls, prob = C.c_int(), C.c_float()
+        check(lib.fl_select_schedule(self.mlp.handle if self.mlp is not None else None, m, n, nnz, dcolptr.ptr,
+                                     drowidx.ptr if nnz else None, C.byref(cls), C.byref(prob),
+                                     stream.ptr if stream is not None else None), "fl_select_schedule")
+        c, p = int(cls.value), float(prob.value)
+        if p < self.min_prob:
+            c = DEFAULT_CLASS
+        return c, p, features(m, n, dcolptr, drowidx, stream)
 
     def choose_refined(self, m, n, dcolptr, drowidx, dval, top: int = 3, reps: int = 10):
         """Model-guided tuning: the network ranks the schedules, the `top` best-ranked are timed

@@ -203,3 +203,18 @@ def test_refined_choice_times_the_top_ranked(gpu):
     run.step(dy)
     yo, S = ob.spmv_oracle(m, n, cp, ri, v.astype(np.float32), x)
     assert rel_err(dy.to_host().astype(np.float64), yo, S) <= 1e-5
+
+
+def test_native_selection_matches_features_and_network(gpu):
+    s = sel.Selector()
+    for case in range(5):
+        m, n, cp, ri, v = list(feature_cases())[case]
+        dcp, dri = DeviceArray.from_host(cp), DeviceArray.from_host(ri)
+        c, p, f = s.choose(m, n, dcp, dri)
+        cls, prob = s.mlp.predict(f[None, :])
+        assert c == int(cls[0]) and abs(p - float(prob[0])) <= 1e-6
+    # no network: the reference schedule, probability 0
+    import ctypes as C
+    out, pr = C.c_int(7), C.c_float(1.0)
+    assert fl.lib.fl_select_schedule(None, m, n, len(ri), dcp.ptr, dri.ptr, C.byref(out), C.byref(pr), None) == 0
+    assert out.value == 0 and pr.value == 0.0

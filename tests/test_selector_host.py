@@ -214,3 +214,32 @@ def test_trained_network_gains_over_default_and_fixed_schedule():
     pick = top3[np.arange(len(te)), np.argmin(np.take_along_axis(T[te], top3, axis=1), axis=1)]
     tuned = sel.evaluate(pick, T[te])
     assert tuned["geomean_regret"] <= learned["geomean_regret"] and tuned["exact_best"] >= learned["exact_best"]
+
+
+def test_selector_entry_points_reject_bad_arguments():
+    """Host-side validation of the selector's C entry points (no device work is started)."""
+    import ctypes as C
+    lib = fl.lib
+    out = np.zeros(40, np.float32)
+    assert lib.fl_features(-1, 5, 0, None, None, out.ctypes.data, None) == -8  # FL_ERR_INVALID
+    assert lib.fl_features(5, 5, 3, None, None, out.ctypes.data, None) == -8
+    h = C.c_void_p()
+    assert lib.fl_mlp_create(C.byref(h), 0, 64, 64, 12, 0) == -8
+    assert lib.fl_mlp_create(None, 40, 64, 64, 12, 0) == -8
+    assert lib.fl_mlp_param_count(None) == 0
+    assert lib.fl_mlp_forward(None, None, 1, None, None) == -8
+    assert lib.fl_mlp_predict(None, None, 1, None, None, None) == -8
+    assert lib.fl_mlp_gradients(None, None, None, 1, None, None, None) == -8
+    assert lib.fl_mlp_train_step(None, None, None, 1, C.c_float(0.1), None, None) == -8
+    assert b"fl_mlp_train_step" in lib.fl_last_error()
+
+
+def test_schedule_table_comes_from_the_library():
+    import ctypes as C
+    names = [s.name for s in sel.SCHEDULES]
+    assert names == ["dropin_w32", "dropin_w64", "fast_w64", "win64", "win128", "win256", "pb128_r4k", "pb128_r16k",
+                     "pb256_r4k", "pb256_r8k", "pb256_r16k", "row"]
+    w = C.c_int()
+    assert fl.lib.fl_schedule_info(12, C.byref(w), None, None, None) == -8
+    assert fl.lib.fl_schedule_info(-1, None, None, None, None) == -8
+    assert fl.lib.fl_select_schedule(None, 5, 5, 0, None, None, None, None, None) == -8
