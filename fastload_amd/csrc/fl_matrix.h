@@ -38,6 +38,7 @@ struct fl_matrix {
     // WIN: rows of each batch compacted to LDS slots.
     //   win_slot[g]  slot of entry g inside its batch; win_rows[win_off[b] + s] = row of slot s
     bool win_ready = false;
+    int win_maxslots = 0;  // most distinct rows (LDS slots) of any batch
     fl::DevBuf<uint16_t> win_slot;
     fl::DevBuf<int> win_rows, win_off;
     // PB (two-phase propagation blocking): rows split in bands of pb_R rows.

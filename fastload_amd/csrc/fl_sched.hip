@@ -5,9 +5,10 @@
 //
 // The tiles are re-ordered (stable) by the 256-column bucket of their first sorted column, so
 // tiles of different levels that read the same part of x run in the same batch. A batch is `bt`
-// consecutive re-ordered tiles (<= bt*64 entries), handled by one 1024-thread workgroup; each
-// of its 16 waves processes bt/16 tiles, reading 64 consecutive sorted x values per tile as in
-// FastLoad, from a window of x that stays in L1/L2 instead of being re-streamed per level.
+// consecutive re-ordered tiles (<= bt*64 entries), handled by one workgroup whose waves take 16
+// tiles each (bt*4 threads; WIN caps it at 512), reading 64 consecutive sorted x values per
+// tile as in FastLoad, from a window of x that stays in L1/L2 instead of being re-streamed per
+// level.
 //
 // WIN (rows with locality: stencils, bands, blocks): the distinct rows of a batch are
 //   compacted to LDS slots at preprocessing time; entries store a 2-byte slot instead of a
@@ -165,6 +166,10 @@ __global__ void k_run_len(const int *__restrict__ run_pos, int nruns, int total,
     GRID_STRIDE(r, nruns) run_len[r] = ((r + 1 < nruns) ? run_pos[r + 1] : total) - run_pos[r];
 }
 
+__global__ void k_adjdiff(const int *__restrict__ in, int count, int *__restrict__ out) {
+    GRID_STRIDE(i, count) out[i] = in[i + 1] - in[i];
+}
+
 // Every entry's destination must be its run's destination plus its offset inside the run (the
 // phase-1 copy moves whole runs).
 __global__ void k_check_runs(const int *__restrict__ orderB, const int *__restrict__ head,
@@ -268,6 +273,7 @@ void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
     if (nnz == 0) {
         A->win_rows.alloc(1);
         FL_HIP(hipMemsetAsync(A->win_off.p, 0, sizeof(int) * ((size_t)nb + 1), s));
+        A->win_maxslots = 0;
         A->win_ready = true;
         return;
     }
@@ -301,6 +307,10 @@ void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
         set_error("WIN schedule: preprocessing produced out-of-range slots");
         throw HipError{hipErrorInvalidValue};
     }
+    DevBuf<int> per_batch(nb);
+    k_adjdiff<<<sg(nb), kBlock, 0, s>>>(A->win_off.p, nb, per_batch.p);
+    FL_LAUNCH_CHECK();
+    A->win_maxslots = device_max_i32(per_batch.p, nb, s);
     A->win_ready = true;
 }
 
@@ -422,10 +432,10 @@ struct OrdTiles {
 
 // Per-wave tile loop over this batch: the wave's (<= 16) tiles' metadata is loaded once (lane j
 // holds tile j), then U tiles at a time are handed to fn(ptr[], cx[], len[], lane).
-template <int U, typename F>
+template <int U, int NT = kBigBlock, typename F>
 __device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batch, F &&fn) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tpw = bt / (kBigBlock / 64);
+    const int tpw = bt / (NT / 64);
     const int t0 = batch * bt + wave * tpw;
     const int t1 = min(t0 + tpw, ot.T);
     if (t0 >= t1) return;
@@ -449,26 +459,26 @@ __device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batc
     }
 }
 
-template <typename V, typename X, typename Y, bool SPLIT, int U, int BT>
-__global__ __launch_bounds__(kBigBlock) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
-                                                   const V *__restrict__ val, const V *__restrict__ val_lo,
-                                                   const X *__restrict__ x, const int *__restrict__ win_rows,
-                                                   const int *__restrict__ win_off, Y *__restrict__ y) {
+template <typename V, typename X, typename Y, bool SPLIT, int U, int BT, int NT>
+__global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
+                                            const V *__restrict__ val, const V *__restrict__ val_lo,
+                                            const X *__restrict__ x, const int *__restrict__ win_rows,
+                                            const int *__restrict__ win_off, Y *__restrict__ y) {
     extern __shared__ double acc[];
     const int b = blockIdx.x;
     const int s0 = win_off[b], ns = win_off[b + 1] - s0;
-    // prefetch the rows this thread will flush (ns <= BT*64 = K * kBigBlock) so the flush after
-    // the barrier issues its atomics without waiting on loads
-    constexpr int K = BT * 64 / kBigBlock;
+    // prefetch the rows this thread will flush (ns <= BT*64 = K * NT) so the flush after the
+    // barrier issues its atomics without waiting on loads
+    constexpr int K = BT * 64 / NT;
     int frow[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int i = threadIdx.x + k * kBigBlock;
+        const int i = threadIdx.x + k * NT;
         frow[k] = (i < ns) ? win_rows[s0 + i] : 0;
     }
-    for (int i = threadIdx.x; i < ns; i += kBigBlock) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < ns; i += NT) acc[i] = 0.0;
     __syncthreads();
-    batch_tiles<U>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+    batch_tiles<U, NT>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
         int sl[U];
         double p[U];
 #pragma unroll
@@ -485,13 +495,13 @@ __global__ __launch_bounds__(kBigBlock) void k_win(OrdTiles ot, int bt, const ui
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const int i = threadIdx.x + k * kBigBlock;
+        const int i = threadIdx.x + k * NT;
         if (i < ns) gatomic(&y[frow[k]], acc[i]);
     }
 }
 
-template <typename V, typename X, typename P, bool SPLIT, int U>
-__global__ __launch_bounds__(kBigBlock) void k_pb1(OrdTiles ot, int bt, const uint16_t *__restrict__ lslot,
+template <typename V, typename X, typename P, bool SPLIT, int U, int NT = kBigBlock>
+__global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, const uint16_t *__restrict__ lslot,
                                                    const V *__restrict__ val, const V *__restrict__ val_lo,
                                                    const X *__restrict__ x, const int *__restrict__ run_off,
                                                    const int *__restrict__ run_lstart, const int *__restrict__ run_dest,
@@ -500,19 +510,19 @@ __global__ __launch_bounds__(kBigBlock) void k_pb1(OrdTiles ot, int bt, const ui
     P *stage = reinterpret_cast<P *>(stage_raw);
     const int b = blockIdx.x;
     const int wave = threadIdx.x >> 6;
-    // Wave w copies runs r0 + w + 16*j. Prefetch the metadata of its first 64 runs (lane j holds
+    // Wave w copies runs r0 + w + (NT/64)*j. Prefetch the metadata of its first 64 runs (lane j holds
     // run j) before the tile loop so the copy loop only reads registers.
     const int r0 = run_off[b], r1 = run_off[b + 1];
     int mlen = 0, mls = 0, mdst = 0;
     {
-        const int r = r0 + wave + 16 * (int)(threadIdx.x & 63);
+        const int r = r0 + wave + (NT / 64) * (int)(threadIdx.x & 63);
         if (r < r1) {
             mlen = run_len[r];
             mls = run_lstart[r];
             mdst = run_dest[r];
         }
     }
-    batch_tiles<U>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+    batch_tiles<U, NT>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
         int sl[U];
         double p[U];
 #pragma unroll
@@ -535,8 +545,8 @@ __global__ __launch_bounds__(kBigBlock) void k_pb1(OrdTiles ot, int bt, const ui
             const int ls = __builtin_amdgcn_readlane(mls, j), d = __builtin_amdgcn_readlane(mdst, j);
             for (int i = lane; i < len; i += 64) bins[d + i] = stage[ls + i];
         }
-        if (r0 + wave + 16 * (base + 64) >= r1) break;
-        const int r = r0 + wave + 16 * (base + 64 + lane);
+        if (r0 + wave + (NT / 64) * (base + 64) >= r1) break;
+        const int r = r0 + wave + (NT / 64) * (base + 64 + lane);
         mlen = mls = mdst = 0;
         if (r < r1) {
             mlen = run_len[r];
@@ -615,15 +625,23 @@ static void allow_lds(K kern, size_t bytes) {
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+template <typename V, typename X, typename Y, bool SPLIT, int NT>
+static auto win_kernel(int bt) {
+    constexpr int U = 4;
+    return (bt == 256) ? k_win<V, X, Y, SPLIT, U, 256, NT>
+           : (bt == 128) ? k_win<V, X, Y, SPLIT, U, 128, NT> : k_win<V, X, Y, SPLIT, U, 64, NT>;
+}
+
 template <typename V, typename X, typename Y, bool SPLIT>
 static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x, Y *y, hipStream_t s) {
-    constexpr int U = 4;
-    const size_t lds = (size_t)A->bt * 64 * sizeof(double);
-    auto kern = (A->bt == 256) ? k_win<V, X, Y, SPLIT, U, 256>
-                : (A->bt == 128) ? k_win<V, X, Y, SPLIT, U, 128> : k_win<V, X, Y, SPLIT, U, 64>;
+    // LDS for the most distinct rows any batch has (at most bt*64). Workgroups of 4 waves per 64
+    // tiles (16 tiles per wave; 512 threads at most): more, smaller workgroups per CU overlap
+    // the three phases of a batch (slot set-up, accumulation, flush) better than 1024 threads.
+    const size_t lds = (size_t)std::max(A->win_maxslots, 1) * sizeof(double);
+    const int nt = std::min(A->bt * 4, 512);
+    auto kern = (nt == 256) ? win_kernel<V, X, Y, SPLIT, 256>(A->bt) : win_kernel<V, X, Y, SPLIT, 512>(A->bt);
     allow_lds(kern, lds);
-    kern<<<A->nbatches, kBigBlock, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, lo, x, A->win_rows.p,
-                                             A->win_off.p, y);
+    kern<<<A->nbatches, nt, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, lo, x, A->win_rows.p, A->win_off.p, y);
     FL_LAUNCH_CHECK();
 }
 
@@ -631,11 +649,13 @@ template <typename V, typename X, typename P, typename Y, bool SPLIT>
 static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x, P *bins, Y *y, hipStream_t s) {
     constexpr int U = 4;
     const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
-    auto k1 = k_pb1<V, X, P, SPLIT, U>;
+    // 4 waves per 64 tiles of the batch (16 tiles per wave)
+    const int nt = A->bt * 4;
+    auto k1 = (nt == 256) ? k_pb1<V, X, P, SPLIT, U, 256>
+              : (nt == 512) ? k_pb1<V, X, P, SPLIT, U, 512> : k_pb1<V, X, P, SPLIT, U, 1024>;
     allow_lds(k1, std::max<size_t>(lds1, 8));
-    k1<<<A->nbatches, kBigBlock, lds1, s>>>(ord_tiles(A), A->bt, A->pb_lslot.p, val, lo, x,
-                                            A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p,
-                                            bins);
+    k1<<<A->nbatches, nt, lds1, s>>>(ord_tiles(A), A->bt, A->pb_lslot.p, val, lo, x, A->pb_run_off.p,
+                                     A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
     FL_LAUNCH_CHECK();
     const size_t lds2 = (size_t)A->pb_R * sizeof(double);
     auto k2 = k_pb2<P, Y, 8>;
