@@ -556,15 +556,31 @@ __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, const uint16_t 
     }
 }
 
+// Phase-2 vector loads: 4 products (16 B) with their four 16-bit rows (8 B) in FP32 mode, 2
+// products with two rows in FP64 mode.
 template <typename P>
-struct Pair;
+struct PbVec;
 template <>
-struct Pair<float> {
-    using T = float2;
+struct PbVec<float> {
+    static constexpr int N = 4;
+    using V = float4;
+    using R = uint2;
+    static __device__ __forceinline__ void add(double *acc, const V &v, const R &r) {
+        atomicAdd(&acc[r.x & 0xffffu], (double)v.x);
+        atomicAdd(&acc[r.x >> 16], (double)v.y);
+        atomicAdd(&acc[r.y & 0xffffu], (double)v.z);
+        atomicAdd(&acc[r.y >> 16], (double)v.w);
+    }
 };
 template <>
-struct Pair<double> {
-    using T = double2;
+struct PbVec<double> {
+    static constexpr int N = 2;
+    using V = double2;
+    using R = uint32_t;
+    static __device__ __forceinline__ void add(double *acc, const V &v, const R &r) {
+        atomicAdd(&acc[r & 0xffffu], v.x);
+        atomicAdd(&acc[r >> 16], v.y);
+    }
 };
 
 template <typename P, typename Y, int U>
@@ -572,39 +588,33 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
                                                    const P *__restrict__ bins, const uint16_t *__restrict__ rowlocal,
                                                    Y *__restrict__ y) {
     extern __shared__ double acc[];
-    using P2 = typename Pair<P>::T;
+    using T = PbVec<P>;
+    constexpr int N = T::N;
     const int b = blockIdx.x;
     for (int i = threadIdx.x; i < R; i += kBigBlock) acc[i] = 0.0;
     __syncthreads();
     const int i0 = band_start[b], i1 = band_start[b + 1];
-    // pairs of entries: one 2-wide product load and one 4-byte load of two 16-bit rows
-    const int a0 = i0 + (i0 & 1);
-    if (a0 != i0 && threadIdx.x == 0 && i0 < i1) atomicAdd(&acc[rowlocal[i0]], (double)bins[i0]);
-    const int np = (i1 > a0) ? (i1 - a0) >> 1 : 0;
-    const P2 *__restrict__ b2 = reinterpret_cast<const P2 *>(bins + a0);
-    const uint32_t *__restrict__ r2 = reinterpret_cast<const uint32_t *>(rowlocal + a0);
+    // [i0, a0) head and [t0, i1) tail hold fewer than N entries each; [a0, t0) is N-aligned
+    const int a0 = min((i0 + N - 1) / N * N, i1);
+    const int nv = (i1 - a0) / N;
+    const int t0 = a0 + nv * N;
+    if ((int)threadIdx.x < a0 - i0) atomicAdd(&acc[rowlocal[i0 + threadIdx.x]], (double)bins[i0 + threadIdx.x]);
+    if ((int)threadIdx.x < i1 - t0) atomicAdd(&acc[rowlocal[t0 + threadIdx.x]], (double)bins[t0 + threadIdx.x]);
+    const typename T::V *__restrict__ bv = reinterpret_cast<const typename T::V *>(bins + a0);
+    const typename T::R *__restrict__ rv = reinterpret_cast<const typename T::R *>(rowlocal + a0);
     int k = threadIdx.x;
-    for (; k + (U - 1) * kBigBlock < np; k += U * kBigBlock) {
-        P2 v[U];
-        uint32_t rl[U];
+    for (; k + (U - 1) * kBigBlock < nv; k += U * kBigBlock) {
+        typename T::V v[U];
+        typename T::R rl[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            v[u] = b2[k + u * kBigBlock];
-            rl[u] = r2[k + u * kBigBlock];
+            v[u] = bv[k + u * kBigBlock];
+            rl[u] = rv[k + u * kBigBlock];
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            atomicAdd(&acc[rl[u] & 0xffffu], (double)v[u].x);
-            atomicAdd(&acc[rl[u] >> 16], (double)v[u].y);
-        }
+        for (int u = 0; u < U; ++u) T::add(acc, v[u], rl[u]);
     }
-    for (; k < np; k += kBigBlock) {
-        const P2 v = b2[k];
-        const uint32_t rl = r2[k];
-        atomicAdd(&acc[rl & 0xffffu], (double)v.x);
-        atomicAdd(&acc[rl >> 16], (double)v.y);
-    }
-    if (a0 + 2 * np < i1 && threadIdx.x == 0) atomicAdd(&acc[rowlocal[i1 - 1]], (double)bins[i1 - 1]);
+    for (; k < nv; k += kBigBlock) T::add(acc, bv[k], rv[k]);
     __syncthreads();
     const int r0 = b * R, rows = min(R, m - r0);
     for (int r = threadIdx.x; r < rows; r += kBigBlock) y[r0 + r] += (Y)acc[r];
