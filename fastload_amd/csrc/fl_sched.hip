@@ -80,6 +80,18 @@ __global__ void k_key_copy(const int *__restrict__ src, uint32_t *__restrict__ k
     GRID_STRIDE(i, count) key[i] = (uint32_t)(src[i] / div);
 }
 
+// Batch b runs on XCD b % 8 (workgroups are dealt to the 8 XCDs round-robin). Ordering a band's
+// runs by (b % 8, b / 8) puts the runs one XCD writes next to each other, so a cache line
+// shared by two neighbouring runs is usually completed in one L2 instead of two.
+__global__ void k_key_batch_xcd(const int *__restrict__ batch_of, int per_xcd, uint32_t *__restrict__ key,
+                                int count) {
+    GRID_STRIDE(i, count) {
+        const int b = batch_of[i];
+        key[i] = (uint32_t)((b & 7) * per_xcd + (b >> 3));
+    }
+}
+
+
 __global__ void k_key_gather_div(const int *__restrict__ src, const int *__restrict__ order, int div,
                                  uint32_t *__restrict__ key, int count) {
     GRID_STRIDE(i, count) key[i] = (uint32_t)(src[order[i]] / div);
@@ -340,10 +352,11 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     DevBuf<int> order(nnz), dest(nnz), head(nnz), runidx((size_t)nnz + 1);
     // A: band-major order (band, batch, g): stable by batch, then stable by band, so each
     // (batch, band) run occupies one contiguous range of the product array
-    k_key_copy<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, key.p, nnz, 1);
+    const int per_xcd = (nb + 7) / 8;
+    k_key_batch_xcd<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, per_xcd, key.p, nnz);
     k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
     FL_LAUNCH_CHECK();
-    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(nb - 1, 0)), s);
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(8 * per_xcd - 1, 0)), s);
     k_key_gather_div<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, order.p, R, key.p, nnz);
     FL_LAUNCH_CHECK();
     radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)(nbands - 1)), s);
