@@ -2,8 +2,9 @@
 
 Headline workload (`value`): the power-law configuration -- 4,194,304 x 4,194,304 CSC, column
 nnz Zipf (exponent 1.8423 so the mean is 16 with max 50,000; ~67M nnz), rows uniform without
-repeats, values and x N(0,1), seed 42. A step is one y = A*x (y zeroed, then one SpMV) with the
-matrix and x resident in HBM.
+repeats, values and x N(0,1), seed 42. A step is one y = A*x with the matrix and x resident in
+HBM: y cleared by a fill kernel then one accumulating SpMV, or, for the propagation-blocking
+schedule, one SpMV whose second phase writes every row of y (fl_spmv_assign).
 
 Variants (schedules x value storage):
   plain_*  the reference per-tile algorithm on the reference 32-wide layout (drop-in)
@@ -179,12 +180,16 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
             if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
                 M.prepare_schedule(kernel, bt, max(R, 1024))
             ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
+            pb = kernel == fl.KERNEL_PB
 
-            def run(M=M, kernel=kernel, kind=kind, dsx=sx[(W, s_x)], ydev=ydev):
-                M.spmv(dsx, ydev, kernel, kind)
+            def run(M=M, kernel=kernel, kind=kind, dsx=sx[(W, s_x)], ydev=ydev, pb=pb):
+                # y = A x: PB writes every row in its second phase; the others accumulate into a
+                # y cleared by a fill kernel (outside the kernel events)
+                M.spmv(dsx, ydev, kernel, kind, assign=pb)
         fl.synchronize()
         prep_s = time.time() - t0
-        step_ms, kern_ms, kern_med = time_variant(fl, run, ydev, steps, warmup, zero=kernel >= 0)
+        step_ms, kern_ms, kern_med = time_variant(fl, run, ydev, steps, warmup,
+                                                  zero=kernel >= 0 and kernel != fl.KERNEL_PB)
         y = ydev.to_host().astype(np.float64)
         err = float(np.linalg.norm(y - y_ref) / ref_norm) if ref_norm else 0.0
         s_y = 8 if ydev is y64 else 4
@@ -261,8 +266,7 @@ def run_batched(fl, count, seed, steps, warmup):
             B1.permute_x(dX, dsx)
 
             def batch_step():
-                fl.zero_device(dY, s)
-                B1.spmv(dsx, dY, sched.kernel, fl.VAL_F32, stream=s)
+                B1.spmv(dsx, dY, sched.kernel, fl.VAL_F32, stream=s, assign=True)
         else:
             def batch_step():
                 B.spmm(dX, dY, k, fl.VAL_F32, stream=s)

@@ -204,10 +204,12 @@ class FastLoadMatrix:
         fn = lib.fl_permute_x_f64 if dx.dtype == np.float64 else lib.fl_permute_x_f32
         check(fn(self.handle, dx.ptr, dsx.ptr, stream.ptr if stream is not None else None), "fl_permute_x")
 
-    def spmv(self, dsx: DeviceArray, dy: DeviceArray, kernel=KERNEL_DROPIN, val_kind=VAL_F64, work=None, stream=None):
-        """y += A * sortx on the device."""
-        check(lib.fl_spmv(self.handle, kernel, val_kind, dsx.ptr, dy.ptr, work.ptr if work is not None else None,
-                          stream.ptr if stream is not None else None), "fl_spmv")
+    def spmv(self, dsx: DeviceArray, dy: DeviceArray, kernel=KERNEL_DROPIN, val_kind=VAL_F64, work=None, stream=None,
+             assign=False):
+        """y += A * sortx on the device (y = A * sortx with assign=True)."""
+        fn = lib.fl_spmv_assign if assign else lib.fl_spmv
+        check(fn(self.handle, kernel, val_kind, dsx.ptr, dy.ptr, work.ptr if work is not None else None,
+                 stream.ptr if stream is not None else None), "fl_spmv_assign" if assign else "fl_spmv")
 
     def close(self):
         if getattr(self, "handle", None):

@@ -17,7 +17,7 @@ int check_compressed(const int *d_ptr, int count, const int *d_idx, int nnz, int
 void prepare_values(fl_matrix *A, int kind, hipStream_t s);
 void permute_x(const fl_matrix *A, const void *x, void *sx, bool dbl, hipStream_t s);
 void permute_raw(const int *perm, int n, const void *x, void *sx, bool dbl, hipStream_t s);
-int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, hipStream_t s);
+int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s);
 void win_prepare(fl_matrix *A, int bt, hipStream_t s);
 void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s);
 
@@ -275,7 +275,7 @@ int fl_FastLoad_spmv(const char *filename, const slide_matrix *M, int nnz, int c
     upload(dx, x, colA, s);
     // dry run (FastLoad_GPU.h:334-344)
     FL_HIP(hipMemsetAsync(dy.p, 0, sizeof(double) * std::max(rowA, 1), s));
-    int rc = spmv(&A, FL_KERNEL_DROPIN, FL_VAL_F64, dx.p, dy.p, nullptr, s);
+    int rc = spmv(&A, FL_KERNEL_DROPIN, FL_VAL_F64, dx.p, dy.p, nullptr, false, s);
     if (rc) return rc;
     hipEvent_t e0, e1;
     FL_HIP(hipEventCreate(&e0));
@@ -285,7 +285,7 @@ int fl_FastLoad_spmv(const char *filename, const slide_matrix *M, int nnz, int c
     for (int r = 0; r < reps; ++r) {
         FL_HIP(hipMemsetAsync(dy.p, 0, sizeof(double) * std::max(rowA, 1), s));
         FL_HIP(hipEventRecord(e0, s));
-        rc = spmv(&A, FL_KERNEL_DROPIN, FL_VAL_F64, dx.p, dy.p, nullptr, s);
+        rc = spmv(&A, FL_KERNEL_DROPIN, FL_VAL_F64, dx.p, dy.p, nullptr, false, s);
         FL_HIP(hipEventRecord(e1, s));
         FL_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -570,7 +570,14 @@ int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int ba
 int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work, void *stream) {
     FL_API_BEGIN
     if (!A || (A->tilenum && (!d_sortx || !d_y))) return invalid("fl_spmv: NULL");
-    return spmv(A, kernel, val_kind, d_sortx, d_y, d_work, (hipStream_t)stream);
+    return spmv(A, kernel, val_kind, d_sortx, d_y, d_work, false, (hipStream_t)stream);
+    FL_API_END
+}
+
+int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work, void *stream) {
+    FL_API_BEGIN
+    if (!A || (A->tilenum && !d_sortx) || (A->m && !d_y)) return invalid("fl_spmv_assign: NULL");
+    return spmv(A, kernel, val_kind, d_sortx, d_y, d_work, true, (hipStream_t)stream);
     FL_API_END
 }
 

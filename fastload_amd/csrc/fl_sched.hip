@@ -596,7 +596,8 @@ struct PbVec<double> {
     }
 };
 
-template <typename P, typename Y, int U>
+// ASSIGN: y = A x (every row of the band is written, so y needs no clearing); else y += A x.
+template <typename P, typename Y, int U, bool ASSIGN>
 __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
                                                    const P *__restrict__ bins, const uint16_t *__restrict__ rowlocal,
                                                    Y *__restrict__ y) {
@@ -630,7 +631,10 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
     for (; k < nv; k += kBigBlock) T::add(acc, bv[k], rv[k]);
     __syncthreads();
     const int r0 = b * R, rows = min(R, m - r0);
-    for (int r = threadIdx.x; r < rows; r += kBigBlock) y[r0 + r] += (Y)acc[r];
+    for (int r = threadIdx.x; r < rows; r += kBigBlock) {
+        if constexpr (ASSIGN) y[r0 + r] = (Y)acc[r];
+        else y[r0 + r] += (Y)acc[r];
+    }
 }
 
 // ------------------------------------------------------------------------------ launch
@@ -669,7 +673,8 @@ static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x
 }
 
 template <typename V, typename X, typename P, typename Y, bool SPLIT>
-static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x, P *bins, Y *y, hipStream_t s) {
+static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x, P *bins, Y *y, bool assign,
+                      hipStream_t s) {
     constexpr int U = 4;
     const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
     // 4 waves per 64 tiles of the batch (16 tiles per wave)
@@ -681,13 +686,13 @@ static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x,
                                      A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
     FL_LAUNCH_CHECK();
     const size_t lds2 = (size_t)A->pb_R * sizeof(double);
-    auto k2 = k_pb2<P, Y, 8>;
+    auto k2 = assign ? k_pb2<P, Y, 8, true> : k_pb2<P, Y, 8, false>;
     allow_lds(k2, lds2);
     k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p, bins, A->pb_rowlocal.p, y);
     FL_LAUNCH_CHECK();
 }
 
-int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, hipStream_t s) {
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s) {
     const bool win = (kernel == FL_KERNEL_WIN);
     if (win && !A->win_ready) { set_error("fl_spmv: WIN schedule not prepared"); return FL_ERR_INVALID; }
     if (!win && !A->pb_ready) { set_error("fl_spmv: PB schedule not prepared"); return FL_ERR_INVALID; }
@@ -703,19 +708,19 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, hip
     switch (kind) {
         case FL_VAL_F64:
             if (win) launch_win<double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, (double *)y, s);
-            else launch_pb<double, double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, A->pb_bins64.p, (double *)y, s);
+            else launch_pb<double, double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, A->pb_bins64.p, (double *)y, assign, s);
             return FL_OK;
         case FL_VAL_F32:
             if (win) launch_win<float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, (float *)y, s);
-            else launch_pb<float, float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, s);
+            else launch_pb<float, float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16:
             if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, (float *)y, s);
-            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, s);
+            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16X2:
             if (win) launch_win<uint16_t, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, (double *)y, s);
-            else launch_pb<uint16_t, float, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, A->pb_bins32.p, (double *)y, s);
+            else launch_pb<uint16_t, float, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, A->pb_bins32.p, (double *)y, assign, s);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);

@@ -244,14 +244,19 @@ static void launch_fast(const fl_matrix *A, const V *val, const V *val_lo, const
     FL_LAUNCH_CHECK();
 }
 
-int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, hipStream_t s);
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s);
 void zero_fill(void *d, size_t bytes, hipStream_t s);
 
-int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, hipStream_t s) {
+// assign: y = A x instead of y += A x. The PB schedule writes every row once and needs no
+// clearing; the other kernels accumulate, so y is cleared with a fill kernel first.
+int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s) {
+    if (kind < FL_VAL_F64 || kind > FL_VAL_BF16X2) { set_error("fl_spmv: unknown value kind %d", kind); return FL_ERR_INVALID; }
+    const size_t ybytes = (size_t)A->m * ((kind == FL_VAL_F64 || kind == FL_VAL_BF16X2) ? 8 : 4);
+    if (assign && (A->tilenum == 0 || kernel != FL_KERNEL_PB)) zero_fill(y, ybytes, s);
     if (A->tilenum == 0) return FL_OK;
     if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_PB) {
         if (A->W != 64) { set_error("fl_spmv: batched schedules need a 64-wide tile layout"); return FL_ERR_INVALID; }
-        return spmv_batched(A, kernel, kind, x, y, s);
+        return spmv_batched(A, kernel, kind, x, y, assign && kernel == FL_KERNEL_PB, s);
     }
     if (kernel == FL_KERNEL_DROPIN) {
         if (kind == FL_VAL_F64) {

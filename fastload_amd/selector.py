@@ -34,7 +34,7 @@ import numpy as np
 
 from ._lib import check, lib
 from .api import (GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF, GEN_ZIPF_DENSE, KERNEL_DROPIN, KERNEL_FAST,
-                  KERNEL_PB, KERNEL_WIN, VAL_F32, FastLoadMatrix, gen_matrix, zero_device)
+                  KERNEL_PB, KERNEL_WIN, VAL_F32, FastLoadMatrix, gen_matrix)
 from .device import DeviceArray, Event, synchronize
 
 N_FEATURES = 40
@@ -333,8 +333,7 @@ class ScheduledSpMV:
         if self.sched.kernel == KERNEL_ROW:
             self.M.spmm(self.dsx, dy, 1, VAL_F32, accumulate=False, stream=stream)
         else:
-            zero_device(dy, stream)  # a fill kernel, like the SpMV itself enqueued on `stream`
-            self.M.spmv(self.dsx, dy, self.sched.kernel, VAL_F32, stream=stream)
+            self.M.spmv(self.dsx, dy, self.sched.kernel, VAL_F32, stream=stream, assign=True)
 
 
 # ------------------------------------------------------------------ sweep population

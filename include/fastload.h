@@ -199,6 +199,10 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
  *   FL_VAL_F64: x,y double.  FL_VAL_F32/FL_VAL_BF16: x,y float.
  *   FL_VAL_BF16X2: x float; y double (recombined); needs d_work of 2*m floats (zeroed inside). */
 int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work, void *stream);
+/* y = A * sortx (no clearing needed; BLAS beta = 0). FL_KERNEL_PB writes every row of y once in
+ * its second phase; the other kernels clear y with a fill kernel, then accumulate. */
+int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work,
+                   void *stream);
 
 /* Builds (once per parameter set) the preprocessing of a batched schedule on a 64-wide matrix:
  *   kernel FL_KERNEL_WIN: batch_tiles in {64, 128, 256} (LDS = batch_tiles*512 bytes);

@@ -197,3 +197,29 @@ def test_reference_style_driver(gpu, tmp_path):
     r = subprocess.run([str(exe), str(GOLDEN / "1138_bus.mtx")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "Check FASTLOAD GPU PASS!" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("kernel,kind,W", [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_FAST, fl.VAL_F32, 64),
+                                           (fl.KERNEL_WIN, fl.VAL_F32, 64), (fl.KERNEL_PB, fl.VAL_F32, 64),
+                                           (fl.KERNEL_PB, fl.VAL_BF16X2, 64), (fl.KERNEL_PB, fl.VAL_F64, 64)],
+                         ids=["dropin-f64", "fast-f32", "win-f32", "pb-f32", "pb-bf16x2", "pb-f64"])
+@pytest.mark.parametrize("name", ["zipf", "empty_cols", "all_empty", "uniform"])
+def test_spmv_assign_overwrites_y(gpu, kernel, kind, W, name):
+    """fl_spmv_assign computes y = A x whatever y held before (rows without entries become 0)."""
+    from fastload_amd import DeviceArray
+    _, m, n, cp, ri, v, x = next(c for c in CASES if c[0] == name)
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=W)
+    if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+        M.prepare_schedule(kernel, 128, 1024)
+    dbl = kind in (fl.VAL_F64, fl.VAL_BF16X2)
+    if kind != fl.VAL_F64:
+        M.prepare(kind)
+    xdt = np.float64 if kind == fl.VAL_F64 else np.float32
+    dx = DeviceArray.from_host(np.asarray(x, xdt))
+    dsx = DeviceArray(max(n, 1), xdt)
+    M.permute_x(dx, dsx)
+    dy = DeviceArray.from_host(np.full(max(m, 1), 123.0, np.float64 if dbl else np.float32))
+    M.spmv(dsx, dy, kernel, kind, assign=True)
+    y = dy.to_host().astype(np.float64)[:m]
+    y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
+    assert rel_err(y, y_or[:m], scale[:m]) <= TOL[kind]
