@@ -118,11 +118,15 @@ def time_variant(fl, run, dy, steps, warmup, zero=True):
     return step_ms, float(kern.mean()), float(np.median(kern))
 
 
-def select_schedule(fl, m, n, dcp, dri, dv, top=3):
+def select_schedule(fl, m, n, dcp, dri, dv, top=3, force=None):
     """The learned selector on the resident matrix: features and MLP inference on the GPU rank the
     12 schedules; the `top` best-ranked are timed on the matrix (model-guided tuning) and the
-    fastest is used. Both the network's first choice and the tuned one are reported."""
+    fastest is used. Both the network's first choice and the tuned one are reported. `force`
+    (a schedule name) skips the selection, for profiling runs whose timings the profiler skews."""
     from fastload_amd import selector as sel
+    if force:
+        c = next(i for i, sc in enumerate(sel.SCHEDULES) if sc.name == force)
+        return sel.SCHEDULES[c], dict(cls=c, schedule=force, forced=True)
     s = sel.Selector()
     fl.synchronize()
     t0 = time.perf_counter()
@@ -378,6 +382,9 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
     ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
     ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,pb_f32,row_f32")
+    ap.add_argument("--schedule", default=None, help="use this schedule for the learned variants instead of "
+                    "selecting one (profiling runs)")
+    ap.add_argument("--stencil-schedule", default=None, help="the same for the stencil workload")
     ap.add_argument("--learned-kinds", default="f32,bf16x2,bf16",
                     help="value storages timed for the learned schedule")
     args = ap.parse_args()
@@ -395,7 +402,7 @@ def main():
     log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
     dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
     del cp, ri, v
-    sched, chosen = select_schedule(fl, m, n, dcp, dri, dv)
+    sched, chosen = select_schedule(fl, m, n, dcp, dri, dv, force=args.schedule)
     log(f"[powerlaw] selector chose {chosen}")
     variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
     variants.update(learned_variants(sched, args.learned_kinds.split(",")))
@@ -437,7 +444,7 @@ def main():
         sx = fl.gen_vector(fl.VEC_NORMAL, 129, sn)
         dcp, dri, dv = DeviceArray.from_host(scp), DeviceArray.from_host(sri), DeviceArray.from_host(sv)
         del scp, sri, sv
-        ssched, schosen = select_schedule(fl, sm, sn, dcp, dri, dv)
+        ssched, schosen = select_schedule(fl, sm, sn, dcp, dri, dv, force=args.stencil_schedule)
         log(f"[stencil] selector chose {schosen}")
         svars = {k: VARIANTS[k] for k in args.stencil_variants.split(",") if k}
         svars.update(learned_variants(ssched, args.learned_kinds.split(",")))
