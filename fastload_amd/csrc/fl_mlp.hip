@@ -20,6 +20,8 @@
 
 namespace fl {
 
+int check_compressed(const int *d_ptr, int count, const int *d_idx, int nnz, int limit, hipStream_t s);
+
 static unsigned sgm(long long count) { return std::max(1u, std::min<unsigned>(grid_for(count), 65536u)); }
 
 #define MSTRIDE(i, count) \
@@ -66,7 +68,7 @@ __global__ void k_feat_rowcount(const int *__restrict__ rowidx, int nnz, int *__
     MSTRIDE(i, nnz) atomicAdd(&rc[rowidx[i]], 1);
 }
 
-// acc[34] sum rc, acc[35] sum rc^2, acc[36] empty rows, acc[37] max rc (as double via atomicMax on int)
+// acc[34] sum rc, acc[35] sum rc^2, acc[36] empty rows; the largest row count goes to maxrc
 __global__ void k_feat_rows(const int *__restrict__ rc, int m, double *__restrict__ acc, int *__restrict__ maxrc) {
     double s = 0.0, s2 = 0.0, e = 0.0;
     int mx = 0;
@@ -312,6 +314,11 @@ int fl_features(int m, int n, int nnz, const int *d_colptr, const int *d_rowidx,
     FL_API_BEGIN
     if (m < 0 || n < 0 || nnz < 0 || !d_colptr || !features || (nnz && !d_rowidx)) return mlp_invalid("fl_features: bad args");
     hipStream_t s = (hipStream_t)stream;
+    // every index the feature kernels use must be inside the matrix
+    if (check_compressed(d_colptr, n, d_rowidx, nnz, m, s)) {
+        set_error("fl_features: malformed CSC (column pointers not monotone from 0 to nnz, or a row index outside [0, m))");
+        return FL_ERR_RANGE;
+    }
     DevBuf<double> acc(40);
     DevBuf<int> rc(std::max(m, 1)), maxrc(1);
     FL_HIP(hipMemsetAsync(acc.p, 0, sizeof(double) * 40, s));

@@ -218,3 +218,12 @@ def test_native_selection_matches_features_and_network(gpu):
     out, pr = C.c_int(7), C.c_float(1.0)
     assert fl.lib.fl_select_schedule(None, m, n, len(ri), dcp.ptr, dri.ptr, C.byref(out), C.byref(pr), None) == 0
     assert out.value == 0 and pr.value == 0.0
+
+
+def test_features_reject_malformed_csc(gpu):
+    cp = np.array([0, 2, 3], np.int32)
+    with pytest.raises(fl.FastLoadError):  # row index 9 outside a 5-row matrix
+        sel.features(5, 2, DeviceArray.from_host(cp), DeviceArray.from_host(np.array([0, 9, 1], np.int32)))
+    with pytest.raises(fl.FastLoadError):  # column pointers not monotone
+        sel.features(5, 2, DeviceArray.from_host(np.array([0, 3, 2], np.int32)),
+                     DeviceArray.from_host(np.array([0, 1], np.int32)))
