@@ -10,6 +10,7 @@ Variants (schedules x value storage):
   plain_*  the reference per-tile algorithm on the reference 32-wide layout (drop-in)
   fast_*   64-wide tiles, level-table geometry, one global atomic per segment
   win_*    batches of 256 tiles, rows compacted to LDS slots, FP64 LDS sums, one atomic per row
+  winr*    the same kernel on tiles grouped by row window (owned rows written without atomics)
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
   row_*    entries re-ordered by row, lanes own rows, no atomics
   learned_* the schedule the learned selector picks for the matrix: features and MLP inference
@@ -59,6 +60,7 @@ VARIANTS = {
     "pb_bf16": (64, 3, 2, 2, 256, 16384),
     "pb_f64": (64, 3, 0, 8, 256, 16384),
     "row_f32": (64, -1, 1, 4, 0, 0),
+    "winr64_f32": (64, 4, 1, 4, 64, 0),
     # schedule-parameter sweep points
     "win128_f32": (64, 2, 1, 4, 128, 0),
     "win64_f32": (64, 2, 1, 4, 64, 0),
@@ -76,7 +78,7 @@ KIND_SVAL = {0: ("f64", 8), 1: ("f32", 4), 2: ("bf16", 2), 3: ("bf16x2", 4)}
 def learned_variants(sched, names=("f32", "bf16x2", "bf16")):
     """Value-storage variants of the schedule the selector chose: FP32, split hi+lo BF16 and BF16
     (those of `names` that the schedule supports without a work buffer)."""
-    kinds = [1, 3, 2] if sched.kernel in (2, 3) else ([1, 2] if sched.kernel >= 0 else [1])
+    kinds = [1, 3, 2] if sched.kernel in (2, 3, 4) else ([1, 2] if sched.kernel >= 0 else [1])
     return {f"learned_{KIND_SVAL[k][0]}": (sched.width, sched.kernel, k, KIND_SVAL[k][1], sched.batch_tiles,
                                            sched.band_rows) for k in kinds if KIND_SVAL[k][0] in names}
 
@@ -181,7 +183,7 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
         else:
             if kind != fl.VAL_F64:
                 M.prepare(kind)
-            if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+            if kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
                 M.prepare_schedule(kernel, bt, max(R, 1024))
             ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
             pb = kernel == fl.KERNEL_PB
@@ -235,7 +237,7 @@ def run_batched(fl, count, seed, steps, warmup):
     B1 = B if sched.width == 64 else fl.FastLoadMatrix(M_, N_, cp, ri, v, width=sched.width)
     if sched.kernel >= 0:
         B1.prepare(fl.VAL_F32)
-    if sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+    if sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
         B1.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
     res = {"matrices": count, "rows": int(M_), "nnz": nnz, "learned_schedule_k1": chosen}
 
@@ -381,7 +383,7 @@ def main():
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
     ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
-    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,pb_f32,row_f32")
+    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,winr64_f32,pb_f32,row_f32")
     ap.add_argument("--schedule", default=None, help="use this schedule for the learned variants instead of "
                     "selecting one (profiling runs)")
     ap.add_argument("--stencil-schedule", default=None, help="the same for the stencil workload")

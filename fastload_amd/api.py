@@ -16,7 +16,7 @@ from ._lib import FastLoadError, MatrixInfo, SlideMatrix, check, i32p, f64p, lib
 from .device import DeviceArray
 
 VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2 = 0, 1, 2, 3
-KERNEL_DROPIN, KERNEL_FAST, KERNEL_WIN, KERNEL_PB = 0, 1, 2, 3
+KERNEL_DROPIN, KERNEL_FAST, KERNEL_WIN, KERNEL_PB, KERNEL_WINR = 0, 1, 2, 3, 4
 
 GEN_SYMGEOM, GEN_ZIPF, GEN_STENCIL27, GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF_DENSE = range(7)
 VEC_NORMAL, VEC_UNIFORM, VEC_MOD10 = range(3)

@@ -18,7 +18,7 @@ void prepare_values(fl_matrix *A, int kind, hipStream_t s);
 void permute_x(const fl_matrix *A, const void *x, void *sx, bool dbl, hipStream_t s);
 void permute_raw(const int *perm, int n, const void *x, void *sx, bool dbl, hipStream_t s);
 int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s);
-void win_prepare(fl_matrix *A, int bt, hipStream_t s);
+void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s);
 void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s);
 
 template <typename T>
@@ -554,14 +554,15 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream) {
 int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int band_rows, void *stream) {
     FL_API_BEGIN
     if (!A) return invalid("fl_matrix_prepare_schedule: NULL");
-    if (kernel != FL_KERNEL_WIN && kernel != FL_KERNEL_PB) return invalid("fl_matrix_prepare_schedule: kernel must be WIN or PB");
+    if (kernel != FL_KERNEL_WIN && kernel != FL_KERNEL_WINR && kernel != FL_KERNEL_PB)
+        return invalid("fl_matrix_prepare_schedule: kernel must be WIN, WINR or PB");
     if (A->W != 64) return invalid("fl_matrix_prepare_schedule: needs a 64-wide tile layout");
     if (batch_tiles != 64 && batch_tiles != 128 && batch_tiles != 256)
         return invalid("fl_matrix_prepare_schedule: batch_tiles must be 64, 128 or 256");
     if (kernel == FL_KERNEL_PB && (band_rows < 1024 || band_rows > 16384))
         return invalid("fl_matrix_prepare_schedule: band_rows must be in [1024, 16384]");
     hipStream_t s = (hipStream_t)stream;
-    if (kernel == FL_KERNEL_WIN) win_prepare(A, batch_tiles, s);
+    if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR) win_prepare(A, batch_tiles, kernel == FL_KERNEL_WINR, s);
     else pb_prepare(A, batch_tiles, band_rows, s);
     return FL_OK;
     FL_API_END

@@ -33,6 +33,7 @@ struct fl_matrix {
     // (len = min(64, n - cx)).
     int bt = 0, nbatches = 0;
     fl::DevBuf<int> ord_ptr, ord_cx;
+    int ord_kind = -1;  // tile order of ord_ptr/ord_cx: 0 column bucket, 1 row window
     fl::DevBuf<int> batch_of;     // [nnz] batch index of every entry (tile order), built on demand
     fl::DevBuf<int> batch_start;  // [nbatches+1] prefix of batch entry counts
     // WIN: rows of each batch compacted to LDS slots.

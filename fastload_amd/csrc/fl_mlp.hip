@@ -359,7 +359,7 @@ int fl_features(int m, int n, int nnz, const int *d_colptr, const int *d_rowidx,
 // Schedule space of the selector: tile width x reduction strategy x batch/band parameters.
 // Class 0 is the reference schedule (32-wide tiles, segmented sums, atomics per segment).
 static const int kSchedules[FL_SCHEDULES][4] = {
-    {32, FL_KERNEL_DROPIN, 0, 0},   {64, FL_KERNEL_DROPIN, 0, 0},  {64, FL_KERNEL_FAST, 0, 0},
+    {32, FL_KERNEL_DROPIN, 0, 0},   {64, FL_KERNEL_DROPIN, 0, 0},  {64, FL_KERNEL_WINR, 64, 0},
     {64, FL_KERNEL_WIN, 64, 0},     {64, FL_KERNEL_WIN, 128, 0},   {64, FL_KERNEL_WIN, 256, 0},
     {64, FL_KERNEL_PB, 128, 4096},  {64, FL_KERNEL_PB, 128, 16384}, {64, FL_KERNEL_PB, 256, 4096},
     {64, FL_KERNEL_PB, 256, 8192},  {64, FL_KERNEL_PB, 256, 16384}, {64, FL_KERNEL_ROW, 0, 0},

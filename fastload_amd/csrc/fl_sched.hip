@@ -47,6 +47,23 @@ __global__ void k_tile_bucket(const int *__restrict__ tile_colidx, int T, uint32
     GRID_STRIDE(t, T) key[t] = (uint32_t)(tile_colidx[t] >> 8);
 }
 
+// row window (64 rows) of a tile's first entry
+__global__ void k_tile_rowkey(const int *__restrict__ tile_ptr, const int *__restrict__ row, int T,
+                              uint32_t *__restrict__ key) {
+    GRID_STRIDE(t, T) key[t] = (uint32_t)(row[tile_ptr[t]] >> 6);
+}
+
+// slots whose row other batches touch too are flushed with an atomic; the others own their row
+__global__ void k_row_batches(const int *__restrict__ win_rows, int D, int *__restrict__ cnt) {
+    GRID_STRIDE(i, D) atomicAdd(&cnt[win_rows[i]], 1);
+}
+__global__ void k_mark_shared(int *__restrict__ win_rows, int D, const int *__restrict__ cnt) {
+    GRID_STRIDE(i, D) {
+        const int r = win_rows[i];
+        if (cnt[r] > 1) win_rows[i] = (int)((uint32_t)r | 0x80000000u);
+    }
+}
+
 __global__ void k_tile_reorder(const int *__restrict__ order, const int *__restrict__ tile_ptr,
                                const int *__restrict__ tile_colidx, int T, int *__restrict__ ord_ptr,
                                int *__restrict__ ord_cx) {
@@ -241,19 +258,24 @@ static void write1(int *d, int v, hipStream_t s) {
     FL_HIP(hipStreamSynchronize(s));
 }
 
-static void ensure_batches(fl_matrix *A, int bt, hipStream_t s) {
+static void ensure_batches(fl_matrix *A, int bt, int kind, hipStream_t s) {
     const int T = A->tilenum;
-    if (!A->ord_ptr.p) {
-        // stable sort of the tiles by 256-column bucket of their first sorted column
+    if (!A->ord_ptr.p || A->ord_kind != kind) {
+        // stable sort of the tiles by 256-column bucket of their first sorted column (kind 0) or
+        // by the 64-row window of their first entry (kind 1)
         A->ord_ptr.alloc(std::max(T, 1));
         A->ord_cx.alloc(std::max(T, 1));
+        A->ord_kind = kind;
+        A->bt = 0;  // batches follow the tile order: rebuild them
         if (T > 0) {
             DevBuf<uint32_t> key(T);
             DevBuf<int> order(T);
-            k_tile_bucket<<<sg(T), kBlock, 0, s>>>(A->tile_colidx.p, T, key.p);
+            if (kind == 1) k_tile_rowkey<<<sg(T), kBlock, 0, s>>>(A->tile_ptr.p, A->sortrowidx.p, T, key.p);
+            else k_tile_bucket<<<sg(T), kBlock, 0, s>>>(A->tile_colidx.p, T, key.p);
             k_iota2<<<sg(T), kBlock, 0, s>>>(order.p, T);
             FL_LAUNCH_CHECK();
-            radix_sort_pairs(key.p, order.p, T, bits_for((uint32_t)std::max((A->n - 1) >> 8, 0)), s);
+            const int top = kind == 1 ? (A->m - 1) >> 6 : (A->n - 1) >> 8;
+            radix_sort_pairs(key.p, order.p, T, bits_for((uint32_t)std::max(top, 0)), s);
             k_tile_reorder<<<sg(T), kBlock, 0, s>>>(order.p, A->tile_ptr.p, A->tile_colidx.p, T, A->ord_ptr.p,
                                                     A->ord_cx.p);
             FL_LAUNCH_CHECK();
@@ -276,8 +298,8 @@ static void ensure_batches(fl_matrix *A, int bt, hipStream_t s) {
     FL_HIP(hipStreamSynchronize(s));
 }
 
-void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
-    ensure_batches(A, bt, s);
+void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
+    ensure_batches(A, bt, rows ? 1 : 0, s);
     if (A->win_ready) return;
     const int nnz = A->nnz, nb = A->nbatches;
     A->win_slot.alloc(std::max(nnz, 1));
@@ -319,6 +341,14 @@ void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
         set_error("WIN schedule: preprocessing produced out-of-range slots");
         throw HipError{hipErrorInvalidValue};
     }
+    {
+        DevBuf<int> cnt(std::max(A->m, 1));
+        FL_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * std::max(A->m, 1), s));
+        k_row_batches<<<sg(D), kBlock, 0, s>>>(A->win_rows.p, D, cnt.p);
+        k_mark_shared<<<sg(D), kBlock, 0, s>>>(A->win_rows.p, D, cnt.p);
+        FL_LAUNCH_CHECK();
+        FL_HIP(hipStreamSynchronize(s));
+    }
     DevBuf<int> per_batch(nb);
     k_adjdiff<<<sg(nb), kBlock, 0, s>>>(A->win_off.p, nb, per_batch.p);
     FL_LAUNCH_CHECK();
@@ -327,7 +357,7 @@ void win_prepare(fl_matrix *A, int bt, hipStream_t s) {
 }
 
 void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
-    ensure_batches(A, bt, s);
+    ensure_batches(A, bt, 0, s);
     if (A->pb_ready && A->pb_R == R) return;
     const int nnz = A->nnz, nb = A->nbatches, m = A->m;
     const int nbands = std::max((m + R - 1) / R, 1);
@@ -509,7 +539,11 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i = threadIdx.x + k * NT;
-        if (i < ns) gatomic(&y[frow[k]], acc[i]);
+        if (i < ns) {
+            const int r = frow[k];
+            if (r >= 0) y[r] += (Y)acc[i];  // only this batch touches row r
+            else gatomic(&y[r & 0x7fffffff], acc[i]);
+        }
     }
 }
 
@@ -693,8 +727,11 @@ static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x,
 }
 
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s) {
-    const bool win = (kernel == FL_KERNEL_WIN);
-    if (win && !A->win_ready) { set_error("fl_spmv: WIN schedule not prepared"); return FL_ERR_INVALID; }
+    const bool win = (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR);
+    if (win && (!A->win_ready || A->ord_kind != (kernel == FL_KERNEL_WINR ? 1 : 0))) {
+        set_error("fl_spmv: %s schedule not prepared", kernel == FL_KERNEL_WINR ? "WINR" : "WIN");
+        return FL_ERR_INVALID;
+    }
     if (!win && !A->pb_ready) { set_error("fl_spmv: PB schedule not prepared"); return FL_ERR_INVALID; }
     if (kind == FL_VAL_F32 && !A->val32.p) { set_error("fl_spmv: FP32 values not prepared"); return FL_ERR_INVALID; }
     if ((kind == FL_VAL_BF16 || kind == FL_VAL_BF16X2) && !A->bf16_hi.p) {

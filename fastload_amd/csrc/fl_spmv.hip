@@ -254,7 +254,7 @@ int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work,
     const size_t ybytes = (size_t)A->m * ((kind == FL_VAL_F64 || kind == FL_VAL_BF16X2) ? 8 : 4);
     if (assign && (A->tilenum == 0 || kernel != FL_KERNEL_PB)) zero_fill(y, ybytes, s);
     if (A->tilenum == 0) return FL_OK;
-    if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_PB) {
+    if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR || kernel == FL_KERNEL_PB) {
         if (A->W != 64) { set_error("fl_spmv: batched schedules need a 64-wide tile layout"); return FL_ERR_INVALID; }
         return spmv_batched(A, kernel, kind, x, y, assign && kernel == FL_KERNEL_PB, s);
     }

@@ -11,7 +11,7 @@ histogram of column lengths plus row statistics drive the choice.
 Schedule space (tile width x reduction strategy x batch/band parameters):
   0 dropin_w32   reference tiles and segmented sums (32 lanes per tile), atomics per segment
   1 dropin_w64   the same algorithm on 64-wide tiles
-  2 fast_w64     64-wide tiles, level-table geometry, one global atomic per segment head
+  2 winr64       WIN on 64-tile batches of tiles grouped by row window (owned rows, no atomic)
   3-5 win{64,128,256}  batches of 64/128/256 tiles, rows compacted to LDS slots, one atomic/row
   6-10 pb*       propagation blocking, batch 128/256 tiles x band 4K/8K/16K rows, no atomics
   11 row         entries re-ordered by row (CSR), lanes own rows, no atomics (fl_spmm, k=1)
@@ -34,7 +34,7 @@ import numpy as np
 
 from ._lib import check, lib
 from .api import (GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF, GEN_ZIPF_DENSE, KERNEL_DROPIN, KERNEL_FAST,
-                  KERNEL_PB, KERNEL_WIN, VAL_F32, FastLoadMatrix, gen_matrix)
+                  KERNEL_PB, KERNEL_WIN, KERNEL_WINR, VAL_F32, FastLoadMatrix, gen_matrix)
 from .device import DeviceArray, Event, synchronize
 
 N_FEATURES = 40
@@ -54,7 +54,8 @@ class Schedule:
 def _schedule_table():
     """The 12 schedules, read from the library (fl_schedule_info) so C and Python callers share
     one table; names follow the parameters (dropin_w32, win128, pb256_r16k, row, ...)."""
-    kname = {KERNEL_DROPIN: "dropin", KERNEL_FAST: "fast", KERNEL_WIN: "win", KERNEL_PB: "pb", KERNEL_ROW: "row"}
+    kname = {KERNEL_DROPIN: "dropin", KERNEL_FAST: "fast", KERNEL_WIN: "win", KERNEL_WINR: "winr", KERNEL_PB: "pb",
+             KERNEL_ROW: "row"}
     out = []
     for c in range(FL_SCHEDULES):
         w, k, bt, r = C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -62,8 +63,8 @@ def _schedule_table():
         w, k, bt, r = w.value, k.value, bt.value, r.value
         if k in (KERNEL_DROPIN, KERNEL_FAST):
             name = f"{kname[k]}_w{w}"
-        elif k == KERNEL_WIN:
-            name = f"win{bt}"
+        elif k in (KERNEL_WIN, KERNEL_WINR):
+            name = f"{kname[k]}{bt}"
         elif k == KERNEL_PB:
             name = f"pb{bt}_r{r // 1024}k"
         else:
@@ -315,7 +316,7 @@ class ScheduledSpMV:
             M.prepare_rows()
         else:
             M.prepare(VAL_F32)
-            if s.kernel in (KERNEL_WIN, KERNEL_PB):
+            if s.kernel in (KERNEL_WIN, KERNEL_WINR, KERNEL_PB):
                 M.prepare_schedule(s.kernel, s.batch_tiles, max(s.band_rows, 1024))
         self.dsx = None
 

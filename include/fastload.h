@@ -191,6 +191,10 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
  *     rows in FP64 LDS and updates y once per row. No global atomics. */
 #define FL_KERNEL_WIN 2
 #define FL_KERNEL_PB 3
+/*   FL_KERNEL_WINR: the WIN kernel on tiles grouped by the 64-row window of their first entry
+ *     instead of by column: on stencils and bands most rows then belong to a single batch and are
+ *     written without an atomic (WIN flushes such owned rows with a plain read-modify-write). */
+#define FL_KERNEL_WINR 4
 
 /* y += A * sortx (y must be zeroed by the caller for y = A*x, as src/main.cu:288-313 does).
  * FL_KERNEL_WIN/FL_KERNEL_PB need fl_matrix_prepare_schedule first; they write FL_VAL_BF16X2
@@ -205,7 +209,7 @@ int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, 
                    void *stream);
 
 /* Builds (once per parameter set) the preprocessing of a batched schedule on a 64-wide matrix:
- *   kernel FL_KERNEL_WIN: batch_tiles in {64, 128, 256} (LDS = batch_tiles*512 bytes);
+ *   kernel FL_KERNEL_WIN / FL_KERNEL_WINR: batch_tiles in {64, 128, 256} (LDS <= batch_tiles*512 bytes);
  *   kernel FL_KERNEL_PB : batch_tiles as above, band_rows in [1024, 16384] (LDS = band_rows*8 bytes).
  * The PB schedule keeps a product buffer of nnz floats (doubles in FP64 mode) allocated on its
  * first use. */

@@ -11,7 +11,7 @@ from fastload_amd import DeviceArray
 def device_y(M: "fl.FastLoadMatrix", x64, kernel, kind, bt=256, R=16384):
     """y = A x through fl_permute_x_* + fl_spmv; returns float64 host y."""
     m = M.m
-    if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+    if kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
         M.prepare_schedule(kernel, bt, R)
     if kind == fl.VAL_F64:
         dx = DeviceArray.from_host(np.asarray(x64, np.float64))

@@ -74,8 +74,9 @@ def test_spmm_invalid_k(gpu):
 
 
 @pytest.mark.parametrize("kernel,kind", [(fl.KERNEL_PB, fl.VAL_F32), (fl.KERNEL_WIN, fl.VAL_F32),
-                                         (fl.KERNEL_FAST, fl.VAL_F64), (fl.KERNEL_PB, fl.VAL_F64)],
-                         ids=["pb-f32", "win-f32", "fast-f64", "pb-f64"])
+                                         (fl.KERNEL_FAST, fl.VAL_F64), (fl.KERNEL_PB, fl.VAL_F64),
+                                         (fl.KERNEL_WINR, fl.VAL_F32)],
+                         ids=["pb-f32", "win-f32", "fast-f64", "pb-f64", "winr-f32"])
 def test_batch_equals_individual_runs(gpu, kernel, kind):
     mats = small_mats()
     M_, N_, cp, ri, v, ro, co = fl.batch_assemble(mats)

@@ -124,8 +124,9 @@ def test_device_layout_matches_reference_golden(gpu):
 KERNELS = [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_DROPIN, fl.VAL_F64, 64), (fl.KERNEL_DROPIN, fl.VAL_F32, 32),
            (fl.KERNEL_DROPIN, fl.VAL_F32, 64), (fl.KERNEL_FAST, fl.VAL_F64, 64), (fl.KERNEL_FAST, fl.VAL_F32, 64),
            (fl.KERNEL_FAST, fl.VAL_BF16, 64), (fl.KERNEL_FAST, fl.VAL_BF16X2, 64)]
-KERNELS += [(k, v, 64) for k in (fl.KERNEL_WIN, fl.KERNEL_PB) for v in (fl.VAL_F64, fl.VAL_F32, fl.VAL_BF16, fl.VAL_BF16X2)]
-KNAMES = ["dropin", "fast", "win", "pb"]
+KERNELS += [(k, v, 64) for k in (fl.KERNEL_WIN, fl.KERNEL_PB, fl.KERNEL_WINR)
+            for v in (fl.VAL_F64, fl.VAL_F32, fl.VAL_BF16, fl.VAL_BF16X2)]
+KNAMES = ["dropin", "fast", "win", "pb", "winr"]
 KIDS = [f"{KNAMES[k]}-{['f64', 'f32', 'bf16', 'bf16x2'][v]}-w{w}" for k, v, w in KERNELS]
 
 
@@ -170,7 +171,7 @@ def test_kat_9x9_gpu(gpu):
             assert y.tolist() == kat["row_sums"], (W, kernel, kind)
 
 
-@pytest.mark.parametrize("kernel", [fl.KERNEL_WIN, fl.KERNEL_PB], ids=["win", "pb"])
+@pytest.mark.parametrize("kernel", [fl.KERNEL_WIN, fl.KERNEL_PB, fl.KERNEL_WINR], ids=["win", "pb", "winr"])
 @pytest.mark.parametrize("bt,R", [(64, 1024), (128, 2048), (256, 4096), (256, 16384)])
 def test_batched_schedule_params(gpu, kernel, bt, R):
     """Multi-batch / multi-band coverage on matrices small enough to check against the oracle."""
@@ -201,15 +202,16 @@ def test_reference_style_driver(gpu, tmp_path):
 
 @pytest.mark.parametrize("kernel,kind,W", [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_FAST, fl.VAL_F32, 64),
                                            (fl.KERNEL_WIN, fl.VAL_F32, 64), (fl.KERNEL_PB, fl.VAL_F32, 64),
-                                           (fl.KERNEL_PB, fl.VAL_BF16X2, 64), (fl.KERNEL_PB, fl.VAL_F64, 64)],
-                         ids=["dropin-f64", "fast-f32", "win-f32", "pb-f32", "pb-bf16x2", "pb-f64"])
+                                           (fl.KERNEL_PB, fl.VAL_BF16X2, 64), (fl.KERNEL_PB, fl.VAL_F64, 64),
+                                           (fl.KERNEL_WINR, fl.VAL_F32, 64)],
+                         ids=["dropin-f64", "fast-f32", "win-f32", "pb-f32", "pb-bf16x2", "pb-f64", "winr-f32"])
 @pytest.mark.parametrize("name", ["zipf", "empty_cols", "all_empty", "uniform"])
 def test_spmv_assign_overwrites_y(gpu, kernel, kind, W, name):
     """fl_spmv_assign computes y = A x whatever y held before (rows without entries become 0)."""
     from fastload_amd import DeviceArray
     _, m, n, cp, ri, v, x = next(c for c in CASES if c[0] == name)
     M = fl.FastLoadMatrix(m, n, cp, ri, v, width=W)
-    if kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+    if kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
         M.prepare_schedule(kernel, 128, 1024)
     dbl = kind in (fl.VAL_F64, fl.VAL_BF16X2)
     if kind != fl.VAL_F64:
@@ -223,3 +225,23 @@ def test_spmv_assign_overwrites_y(gpu, kernel, kind, W, name):
     y = dy.to_host().astype(np.float64)[:m]
     y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
     assert rel_err(y, y_or[:m], scale[:m]) <= TOL[kind]
+
+
+def test_switching_batch_order_invalidates_the_other_schedule(gpu):
+    """WIN and PB share the column-ordered batches; WINR regroups the tiles by row window, so
+    preparing it retires the others until they are prepared again (and vice versa)."""
+    name, m, n, cp, ri, v, x = next(c for c in CASES if c[0] == "banded")
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
+    assert rel_err(device_y(M, x, fl.KERNEL_WIN, fl.VAL_F64, bt=64), y_or, scale) <= TOL[fl.VAL_F64]
+    assert rel_err(device_y(M, x, fl.KERNEL_WINR, fl.VAL_F64, bt=64), y_or, scale) <= TOL[fl.VAL_F64]
+    from fastload_amd import DeviceArray
+    dx = DeviceArray.from_host(np.asarray(x, np.float64))
+    dsx = DeviceArray(n, np.float64)
+    M.permute_x(dx, dsx)
+    dy = DeviceArray.zeros(m, np.float64)
+    with pytest.raises(fl.FastLoadError):
+        M.spmv(dsx, dy, fl.KERNEL_WIN, fl.VAL_F64)  # WIN's batches were replaced by WINR's
+    assert rel_err(device_y(M, x, fl.KERNEL_PB, fl.VAL_F64, bt=64, R=1024), y_or, scale) <= TOL[fl.VAL_F64]
+    with pytest.raises(fl.FastLoadError):
+        M.spmv(dsx, dy, fl.KERNEL_WINR, fl.VAL_F64)

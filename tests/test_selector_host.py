@@ -116,7 +116,7 @@ def test_schedule_table():
     d = sel.SCHEDULES[sel.DEFAULT_CLASS]
     assert (d.width, d.kernel) == (32, fl.KERNEL_DROPIN)  # the reference schedule
     for s in sel.SCHEDULES:
-        if s.kernel in (fl.KERNEL_WIN, fl.KERNEL_PB):
+        if s.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
             assert s.batch_tiles in (64, 128, 256) and s.width == 64
         if s.kernel == fl.KERNEL_PB:
             assert 1024 <= s.band_rows <= 16384
@@ -237,7 +237,7 @@ def test_selector_entry_points_reject_bad_arguments():
 def test_schedule_table_comes_from_the_library():
     import ctypes as C
     names = [s.name for s in sel.SCHEDULES]
-    assert names == ["dropin_w32", "dropin_w64", "fast_w64", "win64", "win128", "win256", "pb128_r4k", "pb128_r16k",
+    assert names == ["dropin_w32", "dropin_w64", "winr64", "win64", "win128", "win256", "pb128_r4k", "pb128_r16k",
                      "pb256_r4k", "pb256_r8k", "pb256_r16k", "row"]
     w = C.c_int()
     assert fl.lib.fl_schedule_info(12, C.byref(w), None, None, None) == -8
