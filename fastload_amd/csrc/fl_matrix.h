@@ -1,6 +1,8 @@
 // Device-resident FastLoad matrix (internal layout; the public handle is opaque).
 #pragma once
 
+#include <vector>
+
 #include "fl_common.h"
 #include "fl_walk.h"
 
@@ -45,8 +47,15 @@ struct fl_matrix {
     // PB (two-phase propagation blocking): rows split in bands of pb_R rows.
     //   phase 1 stages products in LDS at pb_lslot[g] (band-major inside the batch) and copies
     //   runs (batch, band) to the band-major product array; phase 2 accumulates each band.
-    int pb_R = 0, nbands = 0, nruns = 0;
+    //   The batches are split into pb_chunks consecutive chunks (boundaries at multiples of 8
+    //   batches); chunk c's products occupy band-major positions [band_start[c*nbands],
+    //   band_start[(c+1)*nbands]) and are stored at position - pb_chunk_off[c], so every chunk
+    //   reuses the front of the product array and a chunk's products can stay in the Infinity
+    //   Cache between its two phases.
+    int pb_R = 0, nbands = 0, nruns = 0, pb_chunks = 1;
     bool pb_ready = false;
+    std::vector<int> pb_chunk_batch;  // [pb_chunks+1] first batch of every chunk
+    std::vector<int> pb_chunk_off;    // [pb_chunks] product-array offset (a multiple of 4)
     fl::DevBuf<uint16_t> pb_lslot, pb_rowlocal;
     fl::DevBuf<int> pb_band_start, pb_run_off, pb_run_lstart, pb_run_dest, pb_run_len;
     fl::DevBuf<float> pb_bins32;

@@ -109,9 +109,18 @@ __global__ void k_key_batch_xcd(const int *__restrict__ batch_of, int per_xcd, u
 }
 
 
-__global__ void k_key_gather_div(const int *__restrict__ src, const int *__restrict__ order, int div,
-                                 uint32_t *__restrict__ key, int count) {
-    GRID_STRIDE(i, count) key[i] = (uint32_t)(src[order[i]] / div);
+// Chunk-major band key of the entry at sorted position i: chunk * nbands + band, where the
+// chunk is the number of chunk starts cb[1..C-1] at or below the entry's batch.
+__global__ void k_key_chunk_band(const int *__restrict__ batch_of, const int *__restrict__ row,
+                                 const int *__restrict__ order, int R, int nbands, const int *__restrict__ cb,
+                                 int C, uint32_t *__restrict__ key, int count) {
+    GRID_STRIDE(i, count) {
+        const int g = order[i];
+        const int b = batch_of[g];
+        int c = 0;
+        for (int k = 1; k < C; ++k) c += (b >= cb[k]);
+        key[i] = (uint32_t)c * (uint32_t)nbands + (uint32_t)(row[g] / R);
+    }
 }
 
 __global__ void k_key_batch_band(const int *__restrict__ batch_of, const int *__restrict__ row, int R, int nbands,
@@ -145,14 +154,22 @@ __global__ void k_win_slot(const uint32_t *__restrict__ kb, const int *__restric
     }
 }
 
-// PB: dest[g] = band-major position; rowlocal[pos] = row - band*R
-__global__ void k_pb_dest(const int *__restrict__ orderA, const int *__restrict__ row, int R, int *__restrict__ dest,
-                          uint16_t *__restrict__ rowlocal, int count) {
+// PB: dest[g] = product-array position (band-major position minus its chunk's offset);
+// rowlocal[pos] = row - band*R at the band-major position
+__global__ void k_pb_dest(const int *__restrict__ orderA, const int *__restrict__ row, int R,
+                          const uint32_t *__restrict__ key, int nbands, const int *__restrict__ chunk_off,
+                          int *__restrict__ dest, uint16_t *__restrict__ rowlocal, int count) {
     GRID_STRIDE(i, count) {
         const int g = orderA[i];
-        dest[g] = (int)i;
+        dest[g] = (int)i - chunk_off[key[i] / (uint32_t)nbands];
         rowlocal[i] = (uint16_t)(row[g] % R);
     }
+}
+
+// chunk_off[c] = first band-major position of chunk c, rounded down to a multiple of 4 (keeps
+// the 16-byte alignment of phase 2's vector loads in the reused product array)
+__global__ void k_chunk_off(const int *__restrict__ band_start, int nbands, int C, int *__restrict__ chunk_off) {
+    GRID_STRIDE(c, C) chunk_off[c] = band_start[(size_t)c * nbands] & ~3;
 }
 
 // start[b] = first position of key b in the sorted keys (lower bound); start[nb] = count.
@@ -228,7 +245,7 @@ __global__ void k_check_win(const int *__restrict__ batch_of, const uint16_t *__
 }
 
 __global__ void k_check_pb(const int *__restrict__ batch_of, const uint16_t *__restrict__ lslot,
-                           const int *__restrict__ batch_start, int cap, int nnz,
+                           const int *__restrict__ batch_start, int cap, int nnz, int ring,
                            const int *__restrict__ run_off, const int *__restrict__ run_lstart,
                            const int *__restrict__ run_dest, const int *__restrict__ run_len, int nb,
                            int *__restrict__ flag) {
@@ -241,7 +258,7 @@ __global__ void k_check_pb(const int *__restrict__ batch_of, const uint16_t *__r
         const int bsize = batch_start[b + 1] - batch_start[b];
         for (int r = run_off[b]; r < run_off[b + 1]; ++r)
             if (run_len[r] <= 0 || run_lstart[r] < 0 || run_lstart[r] + run_len[r] > bsize || run_dest[r] < 0 ||
-                run_dest[r] + run_len[r] > nnz)
+                run_dest[r] + run_len[r] > ring)
                 atomicOr(flag, 8);
     }
 }
@@ -356,6 +373,27 @@ void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
     A->win_ready = true;
 }
 
+// Chunk boundaries: C consecutive batch ranges of about nnz/C entries each, cut at multiples of
+// 8 batches so that batch b still runs on XCD b % 8 when a chunk's phase 1 starts at its first
+// batch. Empty ranges are dropped.
+static std::vector<int> pb_chunk_batches(const fl_matrix *A, int C, hipStream_t s) {
+    const int nb = A->nbatches;
+    std::vector<int> cb{0};
+    if (C > 1) {
+        std::vector<int> bs((size_t)nb + 1);
+        FL_HIP(hipMemcpyAsync(bs.data(), A->batch_start.p, sizeof(int) * ((size_t)nb + 1), hipMemcpyDeviceToHost, s));
+        FL_HIP(hipStreamSynchronize(s));
+        for (int c = 1; c < C; ++c) {
+            const long long target = (long long)bs[nb] * c / C;
+            int b = (int)(std::lower_bound(bs.begin(), bs.end(), (int)target) - bs.begin());
+            b &= ~7;
+            if (b > cb.back() && b < nb) cb.push_back(b);
+        }
+    }
+    cb.push_back(nb);
+    return cb;
+}
+
 void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     ensure_batches(A, bt, 0, s);
     if (A->pb_ready && A->pb_R == R) return;
@@ -369,30 +407,57 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     A->nbands = nbands;
     A->pb_lslot.alloc(std::max(nnz, 1));
     A->pb_rowlocal.alloc(std::max(nnz, 1));
-    A->pb_band_start.alloc((size_t)nbands + 1);
     A->pb_run_off.alloc((size_t)nb + 1);
     if (nnz == 0) {
+        A->pb_chunks = 1;
+        A->pb_chunk_batch = {0, nb};
+        A->pb_chunk_off = {0};
+        A->pb_band_start.alloc((size_t)nbands + 1);
         FL_HIP(hipMemsetAsync(A->pb_band_start.p, 0, sizeof(int) * ((size_t)nbands + 1), s));
         FL_HIP(hipMemsetAsync(A->pb_run_off.p, 0, sizeof(int) * ((size_t)nb + 1), s));
         A->nruns = 0;
         A->pb_ready = true;
         return;
     }
+    // Two chunks when the FP32 products (4 B each) exceed half of the 256 MB Infinity Cache and y
+    // is small next to them (every chunk after the first re-reads and re-writes y). On the
+    // 67 M-entry power-law matrix two chunks are 3 % faster than one; three or more are slower
+    // (each chunk adds a pass over y and two launch tails).
+    const int want = (nnz >= (1 << 25) && (long long)m * 8 <= nnz) ? 2 : 1;
+    const std::vector<int> cb = pb_chunk_batches(A, want, s);
+    const int C = (int)cb.size() - 1;
+    if ((long long)C * nbands > 0x7fffffffLL) {
+        set_error("PB schedule: %d chunks x %d bands exceed 31-bit keys", C, nbands);
+        throw HipError{hipErrorInvalidValue};
+    }
+    A->pb_band_start.alloc((size_t)C * nbands + 1);
+    DevBuf<int> d_cb(cb.size()), d_off(C);
+    FL_HIP(hipMemcpyAsync(d_cb.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice, s));
     DevBuf<uint32_t> key(nnz);
     DevBuf<int> order(nnz), dest(nnz), head(nnz), runidx((size_t)nnz + 1);
-    // A: band-major order (band, batch, g): stable by batch, then stable by band, so each
-    // (batch, band) run occupies one contiguous range of the product array
+    // A: chunk-major, then band-major order (chunk, band, batch, g): stable by batch (batches in
+    // XCD order), then stable by (chunk, band), so each (batch, band) run occupies one contiguous
+    // range of the product array
     const int per_xcd = (nb + 7) / 8;
     k_key_batch_xcd<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, per_xcd, key.p, nnz);
     k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
     FL_LAUNCH_CHECK();
     radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(8 * per_xcd - 1, 0)), s);
-    k_key_gather_div<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, order.p, R, key.p, nnz);
+    k_key_chunk_band<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->sortrowidx.p, order.p, R, nbands, d_cb.p, C, key.p,
+                                                nnz);
     FL_LAUNCH_CHECK();
-    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)(nbands - 1)), s);
-    k_pb_dest<<<sg(nnz), kBlock, 0, s>>>(order.p, A->sortrowidx.p, R, dest.p, A->pb_rowlocal.p, nnz);
-    k_key_bounds<<<sg((long long)nbands + 1), kBlock, 0, s>>>(key.p, nnz, nbands, A->pb_band_start.p);
+    radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)(C * nbands - 1)), s);
+    k_key_bounds<<<sg((long long)C * nbands + 1), kBlock, 0, s>>>(key.p, nnz, C * nbands, A->pb_band_start.p);
+    k_chunk_off<<<sg(C), kBlock, 0, s>>>(A->pb_band_start.p, nbands, C, d_off.p);
+    k_pb_dest<<<sg(nnz), kBlock, 0, s>>>(order.p, A->sortrowidx.p, R, key.p, nbands, d_off.p, dest.p,
+                                         A->pb_rowlocal.p, nnz);
     FL_LAUNCH_CHECK();
+    std::vector<int> off(C), cstart((size_t)C + 1);
+    FL_HIP(hipMemcpyAsync(off.data(), d_off.p, sizeof(int) * C, hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    int ring = 0;
+    for (int c = 0; c <= C; ++c) cstart[c] = read1(A->pb_band_start.p + (size_t)c * nbands, s);
+    for (int c = 0; c < C; ++c) ring = std::max(ring, cstart[c + 1] - off[c]);
     // B: (batch, band, g) order -> LDS staging slots and runs
     k_key_batch_band<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->sortrowidx.p, R, nbands, key.p, nnz);
     k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
@@ -415,17 +480,19 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     write1(A->pb_run_off.p + nb, nruns, s);
     DevBuf<int> flag(1);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
-    k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->batch_start.p, A->bt * 64, nnz,
+    k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->batch_start.p, A->bt * 64, nnz, ring,
                                           A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, nb,
                                           flag.p);
     k_check_runs<<<sg(nnz), kBlock, 0, s>>>(order.p, head.p, runidx.p, run_pos.p, A->pb_run_dest.p, dest.p, nnz,
                                             flag.p);
     FL_LAUNCH_CHECK();
-    const int total = read1(A->pb_band_start.p + nbands, s);
-    if (read1(flag.p, s) || total != nnz) {
+    if (read1(flag.p, s) || cstart[C] != nnz || ring > nnz) {
         set_error("PB schedule: preprocessing produced out-of-range runs or slots");
         throw HipError{hipErrorInvalidValue};
     }
+    A->pb_chunks = C;
+    A->pb_chunk_batch = cb;
+    A->pb_chunk_off = off;
     A->pb_ready = true;
 }
 
@@ -473,24 +540,37 @@ struct OrdTiles {
     int n, T;
 };
 
-// Per-wave tile loop over this batch: the wave's (<= 16) tiles' metadata is loaded once (lane j
-// holds tile j), then U tiles at a time are handed to fn(ptr[], cx[], len[], lane).
-template <int U, int NT = kBigBlock, typename F>
-__device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batch, F &&fn) {
+// A wave's tiles of one batch: t0..t1 of the re-ordered tiles, with their entry offset (mp)
+// and first column (mc) held by lane t - t0.
+struct WaveTiles {
+    int t0, t1, mp, mc;
+};
+
+template <int NT>
+__device__ __forceinline__ WaveTiles wave_tiles(const OrdTiles &ot, int bt, int batch) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tpw = bt / (NT / 64);
-    const int t0 = batch * bt + wave * tpw;
-    const int t1 = min(t0 + tpw, ot.T);
-    if (t0 >= t1) return;
-    const int mp = (lane < t1 - t0) ? ot.ptr[t0 + lane] : 0;
-    const int mc = (lane < t1 - t0) ? ot.cx[t0 + lane] : 0;
-    for (int t = t0; t < t1; t += U) {
+    WaveTiles w;
+    w.t0 = batch * bt + wave * tpw;
+    w.t1 = min(w.t0 + tpw, ot.T);
+    const bool have = lane < w.t1 - w.t0;
+    w.mp = have ? ot.ptr[w.t0 + lane] : 0;
+    w.mc = have ? ot.cx[w.t0 + lane] : 0;
+    return w;
+}
+
+// Per-wave tile loop over this batch (the wave's <= 16 tiles, metadata from wave_tiles): U tiles
+// at a time are handed to fn(ptr[], cx[], len[], lane).
+template <int U, typename F>
+__device__ __forceinline__ void wave_tile_loop(const OrdTiles &ot, const WaveTiles &w, F &&fn) {
+    const int lane = threadIdx.x & 63;
+    for (int t = w.t0; t < w.t1; t += U) {
         int ptr[U], cx[U], len[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (t + u < t1) {
-                ptr[u] = __builtin_amdgcn_readlane(mp, t + u - t0);
-                cx[u] = __builtin_amdgcn_readlane(mc, t + u - t0);
+            if (t + u < w.t1) {
+                ptr[u] = __builtin_amdgcn_readlane(w.mp, t + u - w.t0);
+                cx[u] = __builtin_amdgcn_readlane(w.mc, t + u - w.t0);
                 len[u] = min(64, ot.n - cx[u]);
             } else {
                 ptr[u] = 0;
@@ -502,13 +582,21 @@ __device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batc
     }
 }
 
-template <typename V, typename X, typename Y, bool SPLIT, int U, int BT, int NT>
+template <int U, int NT = kBigBlock, typename F>
+__device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batch, F &&fn) {
+    const WaveTiles w = wave_tiles<NT>(ot, bt, batch);
+    wave_tile_loop<U>(ot, w, fn);
+}
+
+template <typename V, typename X, typename Y, bool SPLIT, int U, int BT, int NT, bool ASSIGN>
 __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
                                             const V *__restrict__ val, const V *__restrict__ val_lo,
                                             const X *__restrict__ x, const int *__restrict__ win_rows,
                                             const int *__restrict__ win_off, Y *__restrict__ y) {
     extern __shared__ double acc[];
     const int b = blockIdx.x;
+    // tile metadata first: its loads overlap the slot set-up below
+    const WaveTiles w = wave_tiles<NT>(ot, bt, b);
     const int s0 = win_off[b], ns = win_off[b + 1] - s0;
     // prefetch the rows this thread will flush (ns <= BT*64 = K * NT) so the flush after the
     // barrier issues its atomics without waiting on loads
@@ -521,7 +609,7 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
     }
     for (int i = threadIdx.x; i < ns; i += NT) acc[i] = 0.0;
     __syncthreads();
-    batch_tiles<U, NT>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
+    wave_tile_loop<U>(ot, w, [&](const int *ptr, const int *cx, const int *len, int lane) {
         int sl[U];
         double p[U];
 #pragma unroll
@@ -541,21 +629,26 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
         const int i = threadIdx.x + k * NT;
         if (i < ns) {
             const int r = frow[k];
-            if (r >= 0) y[r] += (Y)acc[i];  // only this batch touches row r
+            if (r >= 0) {  // only this batch touches row r
+                if constexpr (ASSIGN) y[r] = (Y)acc[i];
+                else y[r] += (Y)acc[i];
+            }
             else gatomic(&y[r & 0x7fffffff], acc[i]);
         }
     }
 }
 
+// Phase 1 over batches [boff, boff + gridDim.x) (one chunk); boff is a multiple of 8, so batch b
+// runs on XCD b % 8 as the run order of the product array assumes.
 template <typename V, typename X, typename P, bool SPLIT, int U, int NT = kBigBlock>
-__global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, const uint16_t *__restrict__ lslot,
+__global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const uint16_t *__restrict__ lslot,
                                                    const V *__restrict__ val, const V *__restrict__ val_lo,
                                                    const X *__restrict__ x, const int *__restrict__ run_off,
                                                    const int *__restrict__ run_lstart, const int *__restrict__ run_dest,
                                                    const int *__restrict__ run_len, P *__restrict__ bins) {
     extern __shared__ double stage_raw[];
     P *stage = reinterpret_cast<P *>(stage_raw);
-    const int b = blockIdx.x;
+    const int b = boff + blockIdx.x;
     const int wave = threadIdx.x >> 6;
     // Wave w copies runs r0 + w + (NT/64)*j. Prefetch the metadata of its first 64 runs (lane j holds
     // run j) before the tile loop so the copy loop only reads registers.
@@ -631,10 +724,12 @@ struct PbVec<double> {
 };
 
 // ASSIGN: y = A x (every row of the band is written, so y needs no clearing); else y += A x.
+// band_start points at one chunk's band table; the product at band-major position i is
+// bins[i - off] (off a multiple of 4), its row rowlocal[i].
 template <typename P, typename Y, int U, bool ASSIGN>
 __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
-                                                   const P *__restrict__ bins, const uint16_t *__restrict__ rowlocal,
-                                                   Y *__restrict__ y) {
+                                                   const P *__restrict__ bins, int off,
+                                                   const uint16_t *__restrict__ rowlocal, Y *__restrict__ y) {
     extern __shared__ double acc[];
     using T = PbVec<P>;
     constexpr int N = T::N;
@@ -646,9 +741,11 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
     const int a0 = min((i0 + N - 1) / N * N, i1);
     const int nv = (i1 - a0) / N;
     const int t0 = a0 + nv * N;
-    if ((int)threadIdx.x < a0 - i0) atomicAdd(&acc[rowlocal[i0 + threadIdx.x]], (double)bins[i0 + threadIdx.x]);
-    if ((int)threadIdx.x < i1 - t0) atomicAdd(&acc[rowlocal[t0 + threadIdx.x]], (double)bins[t0 + threadIdx.x]);
-    const typename T::V *__restrict__ bv = reinterpret_cast<const typename T::V *>(bins + a0);
+    if ((int)threadIdx.x < a0 - i0)
+        atomicAdd(&acc[rowlocal[i0 + threadIdx.x]], (double)bins[i0 - off + threadIdx.x]);
+    if ((int)threadIdx.x < i1 - t0)
+        atomicAdd(&acc[rowlocal[t0 + threadIdx.x]], (double)bins[t0 - off + threadIdx.x]);
+    const typename T::V *__restrict__ bv = reinterpret_cast<const typename T::V *>(bins + (a0 - off));
     const typename T::R *__restrict__ rv = reinterpret_cast<const typename T::R *>(rowlocal + a0);
     int k = threadIdx.x;
     for (; k + (U - 1) * kBigBlock < nv; k += U * kBigBlock) {
@@ -686,21 +783,25 @@ static void allow_lds(K kern, size_t bytes) {
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-template <typename V, typename X, typename Y, bool SPLIT, int NT>
+template <typename V, typename X, typename Y, bool SPLIT, int NT, bool ASSIGN>
 static auto win_kernel(int bt) {
-    constexpr int U = 4;
-    return (bt == 256) ? k_win<V, X, Y, SPLIT, U, 256, NT>
-           : (bt == 128) ? k_win<V, X, Y, SPLIT, U, 128, NT> : k_win<V, X, Y, SPLIT, U, 64, NT>;
+    constexpr int U = 4;  // tiles in flight per wave (2 and 8 were slower on the 128^3 stencil)
+    return (bt == 256) ? k_win<V, X, Y, SPLIT, U, 256, NT, ASSIGN>
+           : (bt == 128) ? k_win<V, X, Y, SPLIT, U, 128, NT, ASSIGN> : k_win<V, X, Y, SPLIT, U, 64, NT, ASSIGN>;
 }
 
 template <typename V, typename X, typename Y, bool SPLIT>
-static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x, Y *y, hipStream_t s) {
+static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x, Y *y, bool assign,
+                       hipStream_t s) {
     // LDS for the most distinct rows any batch has (at most bt*64). Workgroups of 4 waves per 64
     // tiles (16 tiles per wave; 512 threads at most): more, smaller workgroups per CU overlap
     // the three phases of a batch (slot set-up, accumulation, flush) better than 1024 threads.
     const size_t lds = (size_t)std::max(A->win_maxslots, 1) * sizeof(double);
     const int nt = std::min(A->bt * 4, 512);
-    auto kern = (nt == 256) ? win_kernel<V, X, Y, SPLIT, 256>(A->bt) : win_kernel<V, X, Y, SPLIT, 512>(A->bt);
+    auto kern = (nt == 256) ? (assign ? win_kernel<V, X, Y, SPLIT, 256, true>(A->bt)
+                                      : win_kernel<V, X, Y, SPLIT, 256, false>(A->bt))
+                            : (assign ? win_kernel<V, X, Y, SPLIT, 512, true>(A->bt)
+                                      : win_kernel<V, X, Y, SPLIT, 512, false>(A->bt));
     allow_lds(kern, lds);
     kern<<<A->nbatches, nt, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, lo, x, A->win_rows.p, A->win_off.p, y);
     FL_LAUNCH_CHECK();
@@ -716,14 +817,21 @@ static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x,
     auto k1 = (nt == 256) ? k_pb1<V, X, P, SPLIT, U, 256>
               : (nt == 512) ? k_pb1<V, X, P, SPLIT, U, 512> : k_pb1<V, X, P, SPLIT, U, 1024>;
     allow_lds(k1, std::max<size_t>(lds1, 8));
-    k1<<<A->nbatches, nt, lds1, s>>>(ord_tiles(A), A->bt, A->pb_lslot.p, val, lo, x, A->pb_run_off.p,
-                                     A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
-    FL_LAUNCH_CHECK();
     const size_t lds2 = (size_t)A->pb_R * sizeof(double);
-    auto k2 = assign ? k_pb2<P, Y, 8, true> : k_pb2<P, Y, 8, false>;
-    allow_lds(k2, lds2);
-    k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p, bins, A->pb_rowlocal.p, y);
-    FL_LAUNCH_CHECK();
+    allow_lds(k_pb2<P, Y, 8, true>, lds2);
+    allow_lds(k_pb2<P, Y, 8, false>, lds2);
+    // chunk by chunk: phase 1 writes the chunk's products to the front of the product array,
+    // phase 2 adds them into y (the first chunk assigns y when asked to)
+    for (int c = 0; c < A->pb_chunks; ++c) {
+        const int b0 = A->pb_chunk_batch[c], nbc = A->pb_chunk_batch[c + 1] - b0;
+        k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, A->pb_lslot.p, val, lo, x, A->pb_run_off.p,
+                                 A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
+        FL_LAUNCH_CHECK();
+        auto k2 = (assign && c == 0) ? k_pb2<P, Y, 8, true> : k_pb2<P, Y, 8, false>;
+        k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p + (size_t)c * A->nbands, bins,
+                                              A->pb_chunk_off[c], A->pb_rowlocal.p, y);
+        FL_LAUNCH_CHECK();
+    }
 }
 
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s) {
@@ -744,19 +852,19 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
     }
     switch (kind) {
         case FL_VAL_F64:
-            if (win) launch_win<double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, (double *)y, s);
+            if (win) launch_win<double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, (double *)y, assign, s);
             else launch_pb<double, double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, A->pb_bins64.p, (double *)y, assign, s);
             return FL_OK;
         case FL_VAL_F32:
-            if (win) launch_win<float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, (float *)y, s);
+            if (win) launch_win<float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, (float *)y, assign, s);
             else launch_pb<float, float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16:
-            if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, (float *)y, s);
+            if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, (float *)y, assign, s);
             else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16X2:
-            if (win) launch_win<uint16_t, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, (double *)y, s);
+            if (win) launch_win<uint16_t, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, (double *)y, assign, s);
             else launch_pb<uint16_t, float, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, A->pb_bins32.p, (double *)y, assign, s);
             return FL_OK;
         default:

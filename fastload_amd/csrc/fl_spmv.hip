@@ -248,7 +248,8 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
 void zero_fill(void *d, size_t bytes, hipStream_t s);
 
 // assign: y = A x instead of y += A x. The PB schedule writes every row once and needs no
-// clearing; the other kernels accumulate, so y is cleared with a fill kernel first.
+// clearing; the other kernels accumulate, so y is cleared with a fill kernel first (WIN and
+// WINR then store the rows a single batch owns and add the shared ones atomically).
 int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s) {
     if (kind < FL_VAL_F64 || kind > FL_VAL_BF16X2) { set_error("fl_spmv: unknown value kind %d", kind); return FL_ERR_INVALID; }
     const size_t ybytes = (size_t)A->m * ((kind == FL_VAL_F64 || kind == FL_VAL_BF16X2) ? 8 : 4);
@@ -256,7 +257,7 @@ int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work,
     if (A->tilenum == 0) return FL_OK;
     if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR || kernel == FL_KERNEL_PB) {
         if (A->W != 64) { set_error("fl_spmv: batched schedules need a 64-wide tile layout"); return FL_ERR_INVALID; }
-        return spmv_batched(A, kernel, kind, x, y, assign && kernel == FL_KERNEL_PB, s);
+        return spmv_batched(A, kernel, kind, x, y, assign, s);
     }
     if (kernel == FL_KERNEL_DROPIN) {
         if (kind == FL_VAL_F64) {
