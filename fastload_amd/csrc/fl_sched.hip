@@ -421,7 +421,7 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     }
     // Two chunks when the FP32 products (4 B each) exceed half of the 256 MB Infinity Cache and y
     // is small next to them (every chunk after the first re-reads and re-writes y). On the
-    // 67 M-entry power-law matrix two chunks are 3 % faster than one; three or more are slower
+    // 67 M-entry power-law matrix two chunks are 7 % faster than one; three or more are slower
     // (each chunk adds a pass over y and two launch tails).
     const int want = (nnz >= (1 << 25) && (long long)m * 8 <= nnz) ? 2 : 1;
     const std::vector<int> cb = pb_chunk_batches(A, want, s);
@@ -499,36 +499,6 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
 // ------------------------------------------------------------------------------ kernels
 constexpr int kBigBlock = 1024;
 
-template <typename V>
-struct Vload;
-template <>
-struct Vload<double> {
-    static __device__ __forceinline__ double get(const double *p, int i) { return p[i]; }
-};
-template <>
-struct Vload<float> {
-    static __device__ __forceinline__ float get(const float *p, int i) { return p[i]; }
-};
-template <>
-struct Vload<uint16_t> {
-    static __device__ __forceinline__ float get(const uint16_t *p, int i) { return __uint_as_float((uint32_t)p[i] << 16); }
-};
-
-// Product of entry g with x[cx + lane], in FP64 (FP64 mode) or as the FP64 sum of FP32
-// products (FP32 / BF16 / split hi+lo BF16 storage).
-template <typename V, typename X, bool SPLIT>
-__device__ __forceinline__ double product(const V *__restrict__ val, const V *__restrict__ val_lo,
-                                          const X *__restrict__ x, int g, int xi) {
-    if constexpr (sizeof(X) == 8) {
-        return (double)Vload<V>::get(val, g) * x[xi];
-    } else {
-        const float xv = x[xi];
-        const float p = Vload<V>::get(val, g) * xv;
-        if constexpr (SPLIT) return (double)p + (double)(Vload<V>::get(val_lo, g) * xv);
-        return (double)p;
-    }
-}
-
 __device__ __forceinline__ void gatomic(float *p, double v) { unsafeAtomicAdd(p, (float)v); }
 __device__ __forceinline__ void gatomic(double *p, double v) { unsafeAtomicAdd(p, v); }
 
@@ -559,35 +529,79 @@ __device__ __forceinline__ WaveTiles wave_tiles(const OrdTiles &ot, int bt, int 
     return w;
 }
 
-// Per-wave tile loop over this batch (the wave's <= 16 tiles, metadata from wave_tiles): U tiles
-// at a time are handed to fn(ptr[], cx[], len[], lane).
-template <int U, typename F>
-__device__ __forceinline__ void wave_tile_loop(const OrdTiles &ot, const WaveTiles &w, F &&fn) {
+// Raw operands of U tiles of one wave (value, low part, x, LDS slot; slot -1 = no entry),
+// loaded one group ahead of the group being accumulated.
+template <typename V, typename X, bool SPLIT, int U>
+struct TileGroup {
+    V v[U], lo[U];
+    X xv[U];
+    int sl[U];
+};
+
+template <typename V>
+__device__ __forceinline__ auto widen(V v) {
+    if constexpr (sizeof(V) == 2) return __uint_as_float((uint32_t)v << 16);
+    else return v;
+}
+
+template <typename V, typename X, bool SPLIT, int U>
+__device__ __forceinline__ void load_group(const OrdTiles &ot, const WaveTiles &w, int t,
+                                           const uint16_t *__restrict__ slot, const V *__restrict__ val,
+                                           const V *__restrict__ val_lo, const X *__restrict__ x,
+                                           TileGroup<V, X, SPLIT, U> &G) {
     const int lane = threadIdx.x & 63;
-    for (int t = w.t0; t < w.t1; t += U) {
-        int ptr[U], cx[U], len[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (t + u < w.t1) {
-                ptr[u] = __builtin_amdgcn_readlane(w.mp, t + u - w.t0);
-                cx[u] = __builtin_amdgcn_readlane(w.mc, t + u - w.t0);
-                len[u] = min(64, ot.n - cx[u]);
-            } else {
-                ptr[u] = 0;
-                cx[u] = 0;
-                len[u] = 0;
-            }
+    for (int u = 0; u < U; ++u) {
+        int len = 0, ptr = 0, cx = 0;
+        if (t + u < w.t1) {
+            ptr = __builtin_amdgcn_readlane(w.mp, t + u - w.t0);
+            cx = __builtin_amdgcn_readlane(w.mc, t + u - w.t0);
+            len = min(64, ot.n - cx);
         }
-        fn(ptr, cx, len, lane);
+        const bool act = lane < len;
+        const int g = ptr + lane;
+        G.sl[u] = act ? (int)slot[g] : -1;
+        G.v[u] = act ? val[g] : (V)0;
+        if constexpr (SPLIT) G.lo[u] = act ? val_lo[g] : (V)0;
+        G.xv[u] = act ? x[cx + lane] : (X)0;
     }
 }
 
-template <int U, int NT = kBigBlock, typename F>
-__device__ __forceinline__ void batch_tiles(const OrdTiles &ot, int bt, int batch, F &&fn) {
-    const WaveTiles w = wave_tiles<NT>(ot, bt, batch);
-    wave_tile_loop<U>(ot, w, fn);
+// Product of entry u in FP64 (FP64 mode) or as the FP64 sum of FP32 products (FP32 / BF16 /
+// hi+lo BF16).
+template <typename V, typename X, bool SPLIT, int U>
+__device__ __forceinline__ double group_product(const TileGroup<V, X, SPLIT, U> &G, int u) {
+    if constexpr (sizeof(X) == 8) {
+        return (double)widen(G.v[u]) * G.xv[u];
+    } else {
+        double p = (double)(widen(G.v[u]) * G.xv[u]);
+        if constexpr (SPLIT) p += (double)(widen(G.lo[u]) * G.xv[u]);
+        return p;
+    }
 }
 
+// WIN: add each product to its row's LDS slot.
+template <typename V, typename X, bool SPLIT, int U>
+__device__ __forceinline__ void add_group(double *acc, const TileGroup<V, X, SPLIT, U> &G) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (G.sl[u] >= 0) atomicAdd(&acc[G.sl[u]], group_product(G, u));
+}
+
+// PB phase 1: store each product at its band-major staging slot.
+template <typename V, typename X, bool SPLIT, int U, typename P>
+__device__ __forceinline__ void stage_group(P *stage, const TileGroup<V, X, SPLIT, U> &G) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (G.sl[u] >= 0) stage[G.sl[u]] = (P)group_product(G, u);
+}
+
+// One batch per workgroup: LDS slots of the batch's distinct rows are cleared, every entry adds
+// its product to its row's slot (FP64 LDS atomics), and each slot is flushed to y once.
+// The operands of the next U tiles are loaded before the current U tiles are accumulated, and
+// the first group before the barrier that ends the slot clearing.
+// ASSIGN: y was cleared before the launch and this is its only accumulation, so a row owned by
+// this batch is stored without reading y first (shared rows still take an atomic).
 template <typename V, typename X, typename Y, bool SPLIT, int U, int BT, int NT, bool ASSIGN>
 __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
                                             const V *__restrict__ val, const V *__restrict__ val_lo,
@@ -595,48 +609,44 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
                                             const int *__restrict__ win_off, Y *__restrict__ y) {
     extern __shared__ double acc[];
     const int b = blockIdx.x;
-    // tile metadata first: its loads overlap the slot set-up below
     const WaveTiles w = wave_tiles<NT>(ot, bt, b);
     const int s0 = win_off[b], ns = win_off[b + 1] - s0;
-    // prefetch the rows this thread will flush (ns <= BT*64 = K * NT) so the flush after the
-    // barrier issues its atomics without waiting on loads
+    // prefetch the first rows this thread will flush (ns <= BT*64 = K * NT; most batches have
+    // far fewer distinct rows than entries); the rest are loaded at the flush
     constexpr int K = BT * 64 / NT;
-    int frow[K];
+    constexpr int KP = K < 2 ? K : 2;
+    int frow[KP];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
+    for (int k = 0; k < KP; ++k) {
         const int i = threadIdx.x + k * NT;
         frow[k] = (i < ns) ? win_rows[s0 + i] : 0;
     }
+    TileGroup<V, X, SPLIT, U> cur, nxt;
+    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, slot, val, val_lo, x, cur);
     for (int i = threadIdx.x; i < ns; i += NT) acc[i] = 0.0;
     __syncthreads();
-    wave_tile_loop<U>(ot, w, [&](const int *ptr, const int *cx, const int *len, int lane) {
-        int sl[U];
-        double p[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const bool act = lane < len[u];
-            const int g = ptr[u] + lane;
-            sl[u] = act ? (int)slot[g] : 0;
-            p[u] = act ? product<V, X, SPLIT>(val, val_lo, x, g, cx[u] + lane) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (lane < len[u]) atomicAdd(&acc[sl[u]], p[u]);
-    });
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int i = threadIdx.x + k * NT;
-        if (i < ns) {
-            const int r = frow[k];
-            if (r >= 0) {  // only this batch touches row r
-                if constexpr (ASSIGN) y[r] = (Y)acc[i];
-                else y[r] += (Y)acc[i];
-            }
-            else gatomic(&y[r & 0x7fffffff], acc[i]);
-        }
+    for (int t = w.t0; t < w.t1; t += U) {
+        if (t + U < w.t1) load_group<V, X, SPLIT, U>(ot, w, t + U, slot, val, val_lo, x, nxt);
+        add_group<V, X, SPLIT, U>(acc, cur);
+        cur = nxt;
     }
+    __syncthreads();
+    auto flush = [&](int i, int r) {
+        if (r >= 0) {  // only this batch touches row r
+            if constexpr (ASSIGN) y[r] = (Y)acc[i];
+            else y[r] += (Y)acc[i];
+        } else {
+            gatomic(&y[r & 0x7fffffff], acc[i]);
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const int i = threadIdx.x + k * NT;
+        if (i < ns) flush(i, frow[k]);
+    }
+    for (int i = threadIdx.x + KP * NT; i < ns; i += NT) flush(i, win_rows[s0 + i]);
 }
+
 
 // Phase 1 over batches [boff, boff + gridDim.x) (one chunk); boff is a multiple of 8, so batch b
 // runs on XCD b % 8 as the run order of the product array assumes.
@@ -662,20 +672,15 @@ __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const
             mdst = run_dest[r];
         }
     }
-    batch_tiles<U, NT>(ot, bt, b, [&](const int *ptr, const int *cx, const int *len, int lane) {
-        int sl[U];
-        double p[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const bool act = lane < len[u];
-            const int g = ptr[u] + lane;
-            sl[u] = act ? (int)lslot[g] : 0;
-            p[u] = act ? product<V, X, SPLIT>(val, val_lo, x, g, cx[u] + lane) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (lane < len[u]) stage[sl[u]] = (P)p[u];
-    });
+    // tile loop, operands loaded one group ahead of the group being staged
+    const WaveTiles w = wave_tiles<NT>(ot, bt, b);
+    TileGroup<V, X, SPLIT, U> cur, nxt;
+    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, lslot, val, val_lo, x, cur);
+    for (int t = w.t0; t < w.t1; t += U) {
+        if (t + U < w.t1) load_group<V, X, SPLIT, U>(ot, w, t + U, lslot, val, val_lo, x, nxt);
+        stage_group<V, X, SPLIT, U>(stage, cur);
+        cur = nxt;
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     for (int base = 0;; base += 64) {
@@ -747,17 +752,37 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
         atomicAdd(&acc[rowlocal[t0 + threadIdx.x]], (double)bins[t0 - off + threadIdx.x]);
     const typename T::V *__restrict__ bv = reinterpret_cast<const typename T::V *>(bins + (a0 - off));
     const typename T::R *__restrict__ rv = reinterpret_cast<const typename T::R *>(rowlocal + a0);
-    int k = threadIdx.x;
-    for (; k + (U - 1) * kBigBlock < nv; k += U * kBigBlock) {
-        typename T::V v[U];
-        typename T::R rl[U];
+    // groups of U vectors per thread, each loaded while the previous group is accumulated
+    // (two register sets used in turn)
+    typename T::V va[U], vb[U];
+    typename T::R ra[U], rb[U];
+    auto full = [&](int k) { return k + (U - 1) * kBigBlock < nv; };
+    auto load = [&](int k, typename T::V *v, typename T::R *r) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             v[u] = bv[k + u * kBigBlock];
-            rl[u] = rv[k + u * kBigBlock];
+            r[u] = rv[k + u * kBigBlock];
         }
+    };
+    auto add = [&](const typename T::V *v, const typename T::R *r) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) T::add(acc, v[u], rl[u]);
+        for (int u = 0; u < U; ++u) T::add(acc, v[u], r[u]);
+    };
+    int k = threadIdx.x;
+    if (full(k)) {
+        load(k, va, ra);
+        for (;;) {
+            bool more = full(k + U * kBigBlock);
+            if (more) load(k + U * kBigBlock, vb, rb);
+            add(va, ra);
+            k += U * kBigBlock;
+            if (!more) break;
+            more = full(k + U * kBigBlock);
+            if (more) load(k + U * kBigBlock, va, ra);
+            add(vb, rb);
+            k += U * kBigBlock;
+            if (!more) break;
+        }
     }
     for (; k < nv; k += kBigBlock) T::add(acc, bv[k], rv[k]);
     __syncthreads();
@@ -785,7 +810,7 @@ static void allow_lds(K kern, size_t bytes) {
 
 template <typename V, typename X, typename Y, bool SPLIT, int NT, bool ASSIGN>
 static auto win_kernel(int bt) {
-    constexpr int U = 4;  // tiles in flight per wave (2 and 8 were slower on the 128^3 stencil)
+    constexpr int U = (sizeof(X) == 4 && !SPLIT) ? 8 : 4;  // tiles per group: 8 keeps FP32/BF16 within 64 VGPRs
     return (bt == 256) ? k_win<V, X, Y, SPLIT, U, 256, NT, ASSIGN>
            : (bt == 128) ? k_win<V, X, Y, SPLIT, U, 128, NT, ASSIGN> : k_win<V, X, Y, SPLIT, U, 64, NT, ASSIGN>;
 }
@@ -810,7 +835,7 @@ static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x
 template <typename V, typename X, typename P, typename Y, bool SPLIT>
 static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x, P *bins, Y *y, bool assign,
                       hipStream_t s) {
-    constexpr int U = 4;
+    constexpr int U = (sizeof(X) == 4 && !SPLIT) ? 8 : 4;  // tiles per group, as in WIN
     const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
     // 4 waves per 64 tiles of the batch (16 tiles per wave)
     const int nt = A->bt * 4;
