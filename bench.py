@@ -3,14 +3,16 @@
 Headline workload (`value`): the power-law configuration -- 4,194,304 x 4,194,304 CSC, column
 nnz Zipf (exponent 1.8423 so the mean is 16 with max 50,000; ~67M nnz), rows uniform without
 repeats, values and x N(0,1), seed 42. A step is one y = A*x with the matrix and x resident in
-HBM: y cleared by a fill kernel then one accumulating SpMV, or, for the propagation-blocking
-schedule, one SpMV whose second phase writes every row of y (fl_spmv_assign).
+HBM: y cleared by a fill kernel then one accumulating SpMV (drop-in kernels), or one
+fl_spmv_assign call (batched schedules: the propagation-blocking second phase writes every row
+of y; WIN/WINR clear y and store the rows a single batch owns).
 
 Variants (schedules x value storage):
   plain_*  the reference per-tile algorithm on the reference 32-wide layout (drop-in)
   fast_*   64-wide tiles, level-table geometry, one global atomic per segment
-  win_*    batches of 256 tiles, rows compacted to LDS slots, FP64 LDS sums, one atomic per row
-  winr*    the same kernel on tiles grouped by row window (owned rows written without atomics)
+  win_*    batches of 256 tiles, rows compacted to LDS slots, FP64 LDS sums, one flush per row
+           (a store when no other batch touches the row, else one atomic)
+  winr*    the same kernel on tiles grouped by row window (most rows owned by one batch)
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
   row_*    entries re-ordered by row, lanes own rows, no atomics
   learned_* the schedule the learned selector picks for the matrix: features and MLP inference
@@ -186,16 +188,18 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
             if kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
                 M.prepare_schedule(kernel, bt, max(R, 1024))
             ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
-            pb = kernel == fl.KERNEL_PB
+            assign = kernel in (fl.KERNEL_PB, fl.KERNEL_WIN, fl.KERNEL_WINR)
 
-            def run(M=M, kernel=kernel, kind=kind, dsx=sx[(W, s_x)], ydev=ydev, pb=pb):
-                # y = A x: PB writes every row in its second phase; the others accumulate into a
-                # y cleared by a fill kernel (outside the kernel events)
-                M.spmv(dsx, ydev, kernel, kind, assign=pb)
+            def run(M=M, kernel=kernel, kind=kind, dsx=sx[(W, s_x)], ydev=ydev, assign=assign):
+                # y = A x: fl_spmv_assign for the batched schedules (PB writes every row in its
+                # second phase; WIN/WINR clear y with a fill kernel inside the call, then store the
+                # rows one batch owns); the drop-in kernels accumulate into a y cleared by a fill
+                # kernel outside the kernel events
+                M.spmv(dsx, ydev, kernel, kind, assign=assign)
         fl.synchronize()
         prep_s = time.time() - t0
         step_ms, kern_ms, kern_med = time_variant(fl, run, ydev, steps, warmup,
-                                                  zero=kernel >= 0 and kernel != fl.KERNEL_PB)
+                                                  zero=kernel >= 0 and not assign)
         y = ydev.to_host().astype(np.float64)
         err = float(np.linalg.norm(y - y_ref) / ref_norm) if ref_norm else 0.0
         s_y = 8 if ydev is y64 else 4

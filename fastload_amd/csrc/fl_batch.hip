@@ -179,6 +179,10 @@ __global__ void k_zero(uint32_t *__restrict__ p, long long words) {
     GSTRIDE(i, words) p[i] = 0u;
 }
 
+__global__ void k_zero16(uint4 *__restrict__ p, long long quads) {
+    GSTRIDE(i, quads) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 __global__ void k_zero_bytes(unsigned char *__restrict__ p, long long bytes) {
     GSTRIDE(i, bytes) p[i] = 0;
 }
@@ -186,6 +190,14 @@ __global__ void k_zero_bytes(unsigned char *__restrict__ p, long long bytes) {
 // Zero fill with a kernel (graph-capturable; no memset node).
 void zero_fill(void *d, size_t bytes, hipStream_t s) {
     if (!bytes) return;
+    if ((reinterpret_cast<uintptr_t>(d) & 15) == 0 && bytes >= 16) {  // 16-byte stores for the bulk
+        const long long quads = (long long)(bytes / 16);
+        k_zero16<<<sgb(quads), kBlock, 0, s>>>((uint4 *)d, quads);
+        FL_LAUNCH_CHECK();
+        d = (unsigned char *)d + quads * 16;
+        bytes -= (size_t)quads * 16;
+        if (!bytes) return;
+    }
     const long long words = (long long)(bytes / 4);
     if (words) {
         k_zero<<<sgb(words), kBlock, 0, s>>>((uint32_t *)d, words);

@@ -157,3 +157,17 @@ def test_graph_replays_match_eager(gpu, what):
             assert np.allclose(got, eager, rtol=1e-6, atol=1e-6), rep
         step()  # eager run between replays
         s.synchronize()
+
+
+@pytest.mark.parametrize("off,nbytes", [(0, 4096), (16, 1000 - 16 - 5), (3, 517), (8, 4), (0, 15), (32, 16)])
+def test_zero_fill_exact_range(gpu, off, nbytes):
+    """fl_zero_device clears exactly [ptr, ptr + nbytes): the 16-byte bulk, the 4-byte words and
+    the byte tail, for aligned and unaligned starts."""
+    from fastload_amd._lib import check, lib
+    d = DeviceArray.from_host(np.full(1024, 0xAB, np.uint8))
+    check(lib.fl_zero_device(d.ptr + off, nbytes, None), "fl_zero_device")
+    fl.synchronize()
+    h = d.to_host()
+    want = np.full(1024, 0xAB, np.uint8)
+    want[off:off + nbytes] = 0
+    assert np.array_equal(h, want)
