@@ -16,8 +16,9 @@ Variants (schedules x value storage):
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
   row_*    entries re-ordered by row, lanes own rows, no atomics
   learned_* the schedule the learned selector picks for the matrix: features and MLP inference
-           on the GPU (committed trained weights) rank the 12 schedules, the three best-ranked
-           are timed on the matrix and the fastest is used; in each value storage it supports
+           on the GPU (committed trained weights) rank the 12 schedules, those of probability
+           >= 5 % (at most 5) are timed on the matrix and the fastest is used; in each value
+           storage it supports
   *_f32 / *_f64 / *_bf16 / *_bf16x2: value storage (split hi+lo BF16 recombined in FP64)
 `value` = GB/s-equivalent (algorithmic bytes with FP32 values / step time) of the fastest
 learned_* variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported
@@ -122,11 +123,12 @@ def time_variant(fl, run, dy, steps, warmup, zero=True):
     return step_ms, float(kern.mean()), float(np.median(kern))
 
 
-def select_schedule(fl, m, n, dcp, dri, dv, top=3, force=None):
+def select_schedule(fl, m, n, dcp, dri, dv, top=None, force=None):
     """The learned selector on the resident matrix: features and MLP inference on the GPU rank the
-    12 schedules; the `top` best-ranked are timed on the matrix (model-guided tuning) and the
-    fastest is used. Both the network's first choice and the tuned one are reported. `force`
-    (a schedule name) skips the selection, for profiling runs whose timings the profiler skews."""
+    12 schedules; the schedules the network gives at least 5 % probability (at most 5; or the
+    `top` best-ranked) are timed on the matrix (model-guided tuning) and the fastest is used.
+    Both the network's first choice and the tuned one are reported. `force` (a schedule name)
+    skips the selection, for profiling runs whose timings the profiler skews."""
     from fastload_amd import selector as sel
     if force:
         c = next(i for i, sc in enumerate(sel.SCHEDULES) if sc.name == force)
@@ -140,7 +142,7 @@ def select_schedule(fl, m, n, dcp, dri, dv, top=3, force=None):
     c, times, _ = s.choose_refined(m, n, dcp, dri, dv, top=top)
     ms = (time.perf_counter() - t0) * 1e3
     return sel.SCHEDULES[c], dict(cls=c, schedule=sel.SCHEDULES[c].name, network_choice=sel.SCHEDULES[c1].name,
-                                  network_prob=round(p, 4), select_ms=round(ms1, 3), tuned_top=top,
+                                  network_prob=round(p, 4), select_ms=round(ms1, 3), tuned_candidates=len(times),
                                   tuned_ms={sel.SCHEDULES[k].name: (round(v, 4) if v is not None else None)
                                             for k, v in times.items()},
                                   tuning_s=round(ms / 1e3, 3), trained=s.mlp is not None)

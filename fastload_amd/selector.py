@@ -80,6 +80,10 @@ DEFAULT_CLASS = 0
 
 DATA_DIR = Path(__file__).resolve().parent / "data"
 DATASET_PATH = DATA_DIR / "selector_sweep.npz"
+# Model-guided tuning times every schedule the network gives at least TUNE_MIN_PROB probability
+# (the TUNE_MAX most likely at most, the top one always).
+TUNE_MIN_PROB = 0.05
+TUNE_MAX = 5
 WEIGHTS_PATH = DATA_DIR / "selector_weights.npz"
 
 
@@ -211,6 +215,19 @@ def train(F, times, idx, steps=3000, batch=128, lr=3e-3, tau=0.05, seed=0, log_e
     return mlp, np.array(hist)
 
 
+def softmax(logits):
+    z = np.asarray(logits, np.float64)
+    z = np.exp(z - z.max(axis=-1, keepdims=True))
+    return z / z.sum(axis=-1, keepdims=True)
+
+
+def tuning_candidates(probs, min_prob: float = TUNE_MIN_PROB, max_top: int = TUNE_MAX) -> list[int]:
+    """Classes to time for one matrix: the most likely first, then every other class of
+    probability >= min_prob, at most max_top in all."""
+    order = np.argsort(-np.asarray(probs), kind="stable")
+    return [int(c) for k, c in enumerate(order[:max(max_top, 1)]) if k == 0 or probs[c] >= min_prob]
+
+
 def evaluate(chosen, times):
     """Regret of per-matrix choices against the exhaustive best and the default schedule."""
     t = np.asarray(times, np.float64)
@@ -253,13 +270,19 @@ class Selector:
             return DEFAULT_CLASS, p
         return c, p
 
-    def rank_features(self, f) -> list[int]:
-        """Schedules ordered by the network's preference (the default alone as fallback)."""
+    def probs_features(self, f):
+        """The network's class probabilities, or None when it falls back to the default."""
         f = np.asarray(f, np.float32)
         if self.mlp is None or not np.all(np.isfinite(f)) or f[34] <= 0.0:
+            return None
+        return softmax(self.mlp.forward(f[None, :])[0])
+
+    def rank_features(self, f) -> list[int]:
+        """Schedules ordered by the network's preference (the default alone as fallback)."""
+        p = self.probs_features(f)
+        if p is None:
             return [DEFAULT_CLASS]
-        logits = self.mlp.forward(f[None, :])[0]
-        return [int(c) for c in np.argsort(-logits, kind="stable")]
+        return [int(c) for c in np.argsort(-p, kind="stable")]
 
     def choose(self, m, n, dcolptr, drowidx, stream=None) -> tuple[int, float, np.ndarray]:
         """The network's schedule for a device CSC matrix (fl_select_schedule: features and
@@ -274,12 +297,18 @@ class Selector:
             c = DEFAULT_CLASS
         return c, p, features(m, n, dcolptr, drowidx, stream)
 
-    def choose_refined(self, m, n, dcolptr, drowidx, dval, top: int = 3, reps: int = 10):
-        """Model-guided tuning: the network ranks the schedules, the `top` best-ranked are timed
-        on this matrix (`reps` steps each, after a warm-up step) and the fastest is returned with
-        the measured times. One-time cost: preparing and timing `top` schedules."""
+    def choose_refined(self, m, n, dcolptr, drowidx, dval, top: int | None = None, reps: int = 10):
+        """Model-guided tuning: the network ranks the schedules and the candidates are timed on
+        this matrix (`reps` steps each, after warm-up steps); the fastest is returned with the
+        measured times. Candidates: the `top` best-ranked, or by default (top=None) those of
+        tuning_candidates (probability >= TUNE_MIN_PROB, at most TUNE_MAX). One-time cost:
+        preparing and timing the candidates."""
         f = features(m, n, dcolptr, drowidx)
-        cands = self.rank_features(f)[:max(top, 1)]
+        if top is None:
+            p = self.probs_features(f)
+            cands = [DEFAULT_CLASS] if p is None else tuning_candidates(p)
+        else:
+            cands = self.rank_features(f)[:max(top, 1)]
         if len(cands) == 1:
             return cands[0], {cands[0]: None}, f
         mats, times = {}, {}
@@ -555,6 +584,11 @@ def main(argv=None):
         topk = np.argsort(-logits, axis=1, kind="stable")[:, :k]
         pick = topk[np.arange(len(te)), np.argmin(np.take_along_axis(T[te], topk, axis=1), axis=1)]
         rep[f"learned_top{k}_timed"] = evaluate(pick, T[te])
+    cands = [tuning_candidates(p) for p in softmax(logits)]
+    pick = [c[int(np.argmin(T[te][i, c]))] for i, c in enumerate(cands)]
+    rep["learned_adaptive_timed"] = dict(min_prob=TUNE_MIN_PROB, max_top=TUNE_MAX,
+                                         mean_candidates=float(np.mean([len(c) for c in cands])),
+                                         **evaluate(pick, T[te]))
     rep["chosen_hist"] = {SCHEDULES[c].name: int((cls == c).sum()) for c in range(N_CLASSES)}
     rep["best_hist"] = {SCHEDULES[c].name: int((T[te].argmin(1) == c).sum()) for c in range(N_CLASSES)}
     print(json.dumps(rep, indent=1))

@@ -214,6 +214,25 @@ def test_trained_network_gains_over_default_and_fixed_schedule():
     pick = top3[np.arange(len(te)), np.argmin(np.take_along_axis(T[te], top3, axis=1), axis=1)]
     tuned = sel.evaluate(pick, T[te])
     assert tuned["geomean_regret"] <= learned["geomean_regret"] and tuned["exact_best"] >= learned["exact_best"]
+    # the default tuning rule (every schedule of probability >= 5 %, at most 5) is at least as
+    # good as the top three and times no more schedules than they do on average
+    cands = [sel.tuning_candidates(p) for p in sel.softmax(logits)]
+    assert all(1 <= len(c) <= sel.TUNE_MAX and c[0] == int(np.argmax(lg)) for c, lg in zip(cands, logits))
+    pick = [c[int(np.argmin(T[te][i, c]))] for i, c in enumerate(cands)]
+    adaptive = sel.evaluate(pick, T[te])
+    assert adaptive["geomean_regret"] <= tuned["geomean_regret"] + 1e-9
+    assert np.mean([len(c) for c in cands]) <= 3.0
+
+
+def test_tuning_candidates_rule():
+    """Ordered by probability; the first always, the rest only at >= TUNE_MIN_PROB; at most
+    TUNE_MAX."""
+    p = np.array([0.01, 0.5, 0.049, 0.2, 0.05, 0.06, 0.07, 0.051, 0.0, 0.0, 0.0, 0.0])
+    assert sel.tuning_candidates(p) == [1, 3, 6, 5, 7]
+    assert sel.tuning_candidates(p, max_top=2) == [1, 3]
+    assert sel.tuning_candidates(np.full(12, 1 / 12), min_prob=0.5) == [0]
+    q = sel.softmax(np.array([[1.0, 2.0, 3.0]]))
+    assert np.allclose(q.sum(), 1.0) and np.argmax(q) == 2
 
 
 def test_selector_entry_points_reject_bad_arguments():
