@@ -184,16 +184,18 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
 #define FL_KERNEL_FAST 1
 /*   FL_KERNEL_WIN: batches of consecutive tiles per workgroup; the distinct rows of a batch are
  *     compacted to LDS slots (2-byte slot per entry instead of a 4-byte row), products are summed
- *     per slot in FP64 LDS atomics and each slot is flushed with one global atomic. For rows with
- *     locality (stencils, bands, blocks).
+ *     per slot in FP64 LDS atomics and each slot is flushed once: a row no other batch touches
+ *     with a plain store (fl_spmv_assign) or read-modify-write (fl_spmv), a shared row with one
+ *     global atomic. For rows with locality (stencils, bands, blocks).
  *   FL_KERNEL_PB: two-phase propagation blocking for rows without locality: phase 1 stages the
  *     batch's products in LDS by row band and copies them band-major; phase 2 sums each band of
- *     rows in FP64 LDS and updates y once per row. No global atomics. */
+ *     rows in FP64 LDS and updates y once per row. No global atomics. Matrices of >= 2^25 entries
+ *     run in two chunks of batches so that a chunk's products stay in the Infinity Cache. */
 #define FL_KERNEL_WIN 2
 #define FL_KERNEL_PB 3
 /*   FL_KERNEL_WINR: the WIN kernel on tiles grouped by the 64-row window of their first entry
  *     instead of by column: on stencils and bands most rows then belong to a single batch and are
- *     written without an atomic (WIN flushes such owned rows with a plain read-modify-write). */
+ *     written without an atomic. */
 #define FL_KERNEL_WINR 4
 
 /* y += A * sortx (y must be zeroed by the caller for y = A*x, as src/main.cu:288-313 does).
@@ -204,7 +206,8 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
  *   FL_VAL_BF16X2: x float; y double (recombined); needs d_work of 2*m floats (zeroed inside). */
 int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work, void *stream);
 /* y = A * sortx (no clearing needed; BLAS beta = 0). FL_KERNEL_PB writes every row of y once in
- * its second phase; the other kernels clear y with a fill kernel, then accumulate. */
+ * its second phase; the other kernels clear y with a fill kernel, then accumulate (WIN/WINR
+ * store the rows a single batch owns instead of reading them back). */
 int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work,
                    void *stream);
 
