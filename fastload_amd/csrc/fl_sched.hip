@@ -626,9 +626,10 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
     for (int i = threadIdx.x; i < ns; i += NT) acc[i] = 0.0;
     __syncthreads();
     for (int t = w.t0; t < w.t1; t += U) {
-        if (t + U < w.t1) load_group<V, X, SPLIT, U>(ot, w, t + U, slot, val, val_lo, x, nxt);
+        const bool more = t + U < w.t1;
+        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, slot, val, val_lo, x, nxt);
         add_group<V, X, SPLIT, U>(acc, cur);
-        cur = nxt;
+        if (more) cur = nxt;
     }
     __syncthreads();
     auto flush = [&](int i, int r) {
@@ -677,9 +678,10 @@ __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const
     TileGroup<V, X, SPLIT, U> cur, nxt;
     if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, lslot, val, val_lo, x, cur);
     for (int t = w.t0; t < w.t1; t += U) {
-        if (t + U < w.t1) load_group<V, X, SPLIT, U>(ot, w, t + U, lslot, val, val_lo, x, nxt);
+        const bool more = t + U < w.t1;
+        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, lslot, val, val_lo, x, nxt);
         stage_group<V, X, SPLIT, U>(stage, cur);
-        cur = nxt;
+        if (more) cur = nxt;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
