@@ -245,3 +245,42 @@ def test_switching_batch_order_invalidates_the_other_schedule(gpu):
     assert rel_err(device_y(M, x, fl.KERNEL_PB, fl.VAL_F64, bt=64, R=1024), y_or, scale) <= TOL[fl.VAL_F64]
     with pytest.raises(fl.FastLoadError):
         M.spmv(dsx, dy, fl.KERNEL_WINR, fl.VAL_F64)
+
+
+def _random_case(seed):
+    """A seeded random matrix: family, shape (m != n allowed, down to 1) and parameters drawn."""
+    rng = np.random.default_rng(1000 + seed)
+    m, n = int(rng.integers(1, 3000)), int(rng.integers(1, 3000))
+    fam = seed % 4
+    if fam == 0:
+        m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, seed, m, n, (float(rng.uniform(1.3, 2.5)), int(rng.integers(1, 500))))
+    elif fam == 1:
+        m, n, cp, ri, v = fl.gen_matrix(fl.GEN_UNIFORM, seed, m, n, (float(10 ** rng.uniform(-4, -1)),))
+    elif fam == 2:
+        k = max(1, min(m, n) // int(rng.integers(1, 8)))
+        m, n, cp, ri, v = fl.gen_matrix(fl.GEN_BANDED, seed, k, k, (int(rng.integers(1, 40)),))
+    else:
+        m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF_DENSE, seed, m, n, (2.0, 50, 0.02, int(rng.integers(1, m + 1))))
+    return m, n, cp, ri, v, rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_shapes_layout_and_schedules(gpu, seed):
+    """Seeded random shapes and families: the layout is bit-exact at both widths and every
+    schedule family agrees with the FP64 oracle, at a random batch size and band height."""
+    m, n, cp, ri, v, x = _random_case(seed)
+    rng = np.random.default_rng(seed)
+    y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
+    for W in (32, 64):
+        M = fl.FastLoadMatrix(m, n, cp, ri, v, width=W)
+        bad, _ = export_matches_oracle(M, m, n, cp, ri, v, x, W)
+        assert not bad, (seed, W, bad)
+        runs = [(fl.KERNEL_DROPIN, fl.VAL_F64), (fl.KERNEL_DROPIN, fl.VAL_F32)]
+        if W == 64:
+            runs += [(fl.KERNEL_FAST, fl.VAL_F32), (fl.KERNEL_WIN, fl.VAL_F32), (fl.KERNEL_WINR, fl.VAL_F32),
+                     (fl.KERNEL_PB, fl.VAL_F32), (fl.KERNEL_PB, fl.VAL_F64), (fl.KERNEL_WIN, fl.VAL_BF16X2)]
+        for kernel, kind in runs:
+            bt = int(rng.choice([64, 128, 256]))
+            R = int(rng.choice([1024, 4096, 16384]))
+            y = device_y(M, x, kernel, kind, bt=bt, R=R)
+            assert rel_err(y, y_or, scale) <= TOL[kind], (seed, m, n, W, kernel, kind, bt, R)

@@ -93,3 +93,34 @@ def test_full_size_checksum_and_linearity(gpu, powerlaw):
         assert abs(y1.sum() - want) <= 1e-12 * bound * 10, kernel
         _, s1 = ob.spmv_oracle(m, n, cp, ri, v, np.abs(x) + 2 * np.abs(x2))
         assert rel_err(y12, y1 + 2 * y2, s1) <= 1e-12, kernel
+
+
+@pytest.fixture(scope="module")
+def uniform_42m():
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_UNIFORM, 7, 1 << 20, 1 << 20, (4e-5,))
+    nnz = int(cp[n])
+    assert nnz >= 1 << 25 and m * 8 <= nnz  # large enough for the two-chunk PB
+    return m, n, cp, ri, v, fl.gen_vector(fl.VEC_NORMAL, 8, n)
+
+
+@pytest.mark.parametrize("bt,R", [(64, 1024), (128, 4096), (256, 16384)])
+def test_pb_two_chunks_parity(gpu, uniform_42m, bt, R):
+    """PB on a matrix of >= 2^25 entries runs in two chunks of batches (chunk boundaries at
+    multiples of 8 batches, per-chunk band tables, products at the front of the product array):
+    y against the FP64 oracle for every batch size and band height, and fl_spmv_assign (first
+    chunk writes y, second adds) equal bit for bit to y += A x on a cleared y."""
+    m, n, cp, ri, v, x = uniform_42m
+    y_or, scale = ob.spmv_oracle(m, n, cp, ri, v, x)
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    for kind in (fl.VAL_F32, fl.VAL_F64, fl.VAL_BF16X2):
+        y = device_y(M, x, fl.KERNEL_PB, kind, bt=bt, R=R)
+        assert rel_err(y, y_or, scale) <= TOL[kind], (bt, R, kind)
+    dx = DeviceArray.from_host(x.astype(np.float32))
+    dsx = DeviceArray(n, np.float32)
+    M.permute_x(dx, dsx)
+    dy = DeviceArray.from_host(np.full(m, 7.0, np.float32))
+    M.spmv(dsx, dy, fl.KERNEL_PB, fl.VAL_F32, assign=True)
+    ya = dy.to_host()
+    dy0 = DeviceArray.zeros(m, np.float32)
+    M.spmv(dsx, dy0, fl.KERNEL_PB, fl.VAL_F32)
+    assert np.array_equal(ya, dy0.to_host())
