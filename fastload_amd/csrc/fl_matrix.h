@@ -26,6 +26,7 @@ struct fl_matrix {
     // precision variants (built on demand from val64)
     fl::DevBuf<float> val32;
     fl::DevBuf<uint16_t> bf16_hi, bf16_lo;
+    fl::DevBuf<uint32_t> bf16_pair;  // hi << 16 | lo: one load per entry in the batched schedules
     // FAST kernel: level index of the first tile of every 64-tile group
     fl::DevBuf<int> group_level;
 

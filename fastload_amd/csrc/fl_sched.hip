@@ -529,11 +529,11 @@ __device__ __forceinline__ WaveTiles wave_tiles(const OrdTiles &ot, int bt, int 
     return w;
 }
 
-// Raw operands of U tiles of one wave (value, low part, x, LDS slot; slot -1 = no entry),
+// Raw operands of U tiles of one wave (value or hi+lo BF16 word, x, LDS slot; slot -1 = no entry),
 // loaded one group ahead of the group being accumulated.
 template <typename V, typename X, bool SPLIT, int U>
 struct TileGroup {
-    V v[U], lo[U];
+    V v[U];
     X xv[U];
     int sl[U];
 };
@@ -547,7 +547,7 @@ __device__ __forceinline__ auto widen(V v) {
 template <typename V, typename X, bool SPLIT, int U>
 __device__ __forceinline__ void load_group(const OrdTiles &ot, const WaveTiles &w, int t,
                                            const uint16_t *__restrict__ slot, const V *__restrict__ val,
-                                           const V *__restrict__ val_lo, const X *__restrict__ x,
+                                           const X *__restrict__ x,
                                            TileGroup<V, X, SPLIT, U> &G) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -562,7 +562,6 @@ __device__ __forceinline__ void load_group(const OrdTiles &ot, const WaveTiles &
         const int g = ptr + lane;
         G.sl[u] = act ? (int)slot[g] : -1;
         G.v[u] = act ? val[g] : (V)0;
-        if constexpr (SPLIT) G.lo[u] = act ? val_lo[g] : (V)0;
         G.xv[u] = act ? x[cx + lane] : (X)0;
     }
 }
@@ -573,10 +572,11 @@ template <typename V, typename X, bool SPLIT, int U>
 __device__ __forceinline__ double group_product(const TileGroup<V, X, SPLIT, U> &G, int u) {
     if constexpr (sizeof(X) == 8) {
         return (double)widen(G.v[u]) * G.xv[u];
+    } else if constexpr (SPLIT) {  // hi+lo BF16 pair in one word: hi << 16 | lo
+        const float hi = __uint_as_float(G.v[u] & 0xffff0000u), lo = __uint_as_float(G.v[u] << 16);
+        return (double)(hi * G.xv[u]) + (double)(lo * G.xv[u]);
     } else {
-        double p = (double)(widen(G.v[u]) * G.xv[u]);
-        if constexpr (SPLIT) p += (double)(widen(G.lo[u]) * G.xv[u]);
-        return p;
+        return (double)(widen(G.v[u]) * G.xv[u]);
     }
 }
 
@@ -604,7 +604,7 @@ __device__ __forceinline__ void stage_group(P *stage, const TileGroup<V, X, SPLI
 // this batch is stored without reading y first (shared rows still take an atomic).
 template <typename V, typename X, typename Y, bool SPLIT, int U, int BT, int NT, bool ASSIGN>
 __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
-                                            const V *__restrict__ val, const V *__restrict__ val_lo,
+                                            const V *__restrict__ val,
                                             const X *__restrict__ x, const int *__restrict__ win_rows,
                                             const int *__restrict__ win_off, Y *__restrict__ y) {
     extern __shared__ double acc[];
@@ -622,12 +622,12 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
         frow[k] = (i < ns) ? win_rows[s0 + i] : 0;
     }
     TileGroup<V, X, SPLIT, U> cur, nxt;
-    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, slot, val, val_lo, x, cur);
+    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, slot, val, x, cur);
     for (int i = threadIdx.x; i < ns; i += NT) acc[i] = 0.0;
     __syncthreads();
     for (int t = w.t0; t < w.t1; t += U) {
         const bool more = t + U < w.t1;
-        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, slot, val, val_lo, x, nxt);
+        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, slot, val, x, nxt);
         add_group<V, X, SPLIT, U>(acc, cur);
         if (more) cur = nxt;
     }
@@ -653,7 +653,7 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
 // runs on XCD b % 8 as the run order of the product array assumes.
 template <typename V, typename X, typename P, bool SPLIT, int U, int NT = kBigBlock>
 __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const uint16_t *__restrict__ lslot,
-                                                   const V *__restrict__ val, const V *__restrict__ val_lo,
+                                                   const V *__restrict__ val,
                                                    const X *__restrict__ x, const int *__restrict__ run_off,
                                                    const int *__restrict__ run_lstart, const int *__restrict__ run_dest,
                                                    const int *__restrict__ run_len, P *__restrict__ bins) {
@@ -676,10 +676,10 @@ __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const
     // tile loop, operands loaded one group ahead of the group being staged
     const WaveTiles w = wave_tiles<NT>(ot, bt, b);
     TileGroup<V, X, SPLIT, U> cur, nxt;
-    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, lslot, val, val_lo, x, cur);
+    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, lslot, val, x, cur);
     for (int t = w.t0; t < w.t1; t += U) {
         const bool more = t + U < w.t1;
-        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, lslot, val, val_lo, x, nxt);
+        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, lslot, val, x, nxt);
         stage_group<V, X, SPLIT, U>(stage, cur);
         if (more) cur = nxt;
     }
@@ -812,13 +812,13 @@ static void allow_lds(K kern, size_t bytes) {
 
 template <typename V, typename X, typename Y, bool SPLIT, int NT, bool ASSIGN>
 static auto win_kernel(int bt) {
-    constexpr int U = (sizeof(X) == 4 && !SPLIT) ? 8 : 4;  // tiles per group: 8 keeps FP32/BF16 within 64 VGPRs
+    constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group: 8 keeps the 4-byte-x kinds within 64 VGPRs
     return (bt == 256) ? k_win<V, X, Y, SPLIT, U, 256, NT, ASSIGN>
            : (bt == 128) ? k_win<V, X, Y, SPLIT, U, 128, NT, ASSIGN> : k_win<V, X, Y, SPLIT, U, 64, NT, ASSIGN>;
 }
 
 template <typename V, typename X, typename Y, bool SPLIT>
-static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x, Y *y, bool assign,
+static void launch_win(const fl_matrix *A, const V *val, const X *x, Y *y, bool assign,
                        hipStream_t s) {
     // LDS for the most distinct rows any batch has (at most bt*64). Workgroups of 4 waves per 64
     // tiles (16 tiles per wave; 512 threads at most): more, smaller workgroups per CU overlap
@@ -830,14 +830,14 @@ static void launch_win(const fl_matrix *A, const V *val, const V *lo, const X *x
                             : (assign ? win_kernel<V, X, Y, SPLIT, 512, true>(A->bt)
                                       : win_kernel<V, X, Y, SPLIT, 512, false>(A->bt));
     allow_lds(kern, lds);
-    kern<<<A->nbatches, nt, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, lo, x, A->win_rows.p, A->win_off.p, y);
+    kern<<<A->nbatches, nt, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, x, A->win_rows.p, A->win_off.p, y);
     FL_LAUNCH_CHECK();
 }
 
 template <typename V, typename X, typename P, typename Y, bool SPLIT>
-static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x, P *bins, Y *y, bool assign,
+static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *y, bool assign,
                       hipStream_t s) {
-    constexpr int U = (sizeof(X) == 4 && !SPLIT) ? 8 : 4;  // tiles per group, as in WIN
+    constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group, as in WIN
     const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
     // 4 waves per 64 tiles of the batch (16 tiles per wave)
     const int nt = A->bt * 4;
@@ -851,7 +851,7 @@ static void launch_pb(const fl_matrix *A, const V *val, const V *lo, const X *x,
     // phase 2 adds them into y (the first chunk assigns y when asked to)
     for (int c = 0; c < A->pb_chunks; ++c) {
         const int b0 = A->pb_chunk_batch[c], nbc = A->pb_chunk_batch[c + 1] - b0;
-        k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, A->pb_lslot.p, val, lo, x, A->pb_run_off.p,
+        k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, A->pb_lslot.p, val, x, A->pb_run_off.p,
                                  A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
         FL_LAUNCH_CHECK();
         auto k2 = (assign && c == 0) ? k_pb2<P, Y, 8, true> : k_pb2<P, Y, 8, false>;
@@ -869,7 +869,7 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
     }
     if (!win && !A->pb_ready) { set_error("fl_spmv: PB schedule not prepared"); return FL_ERR_INVALID; }
     if (kind == FL_VAL_F32 && !A->val32.p) { set_error("fl_spmv: FP32 values not prepared"); return FL_ERR_INVALID; }
-    if ((kind == FL_VAL_BF16 || kind == FL_VAL_BF16X2) && !A->bf16_hi.p) {
+    if ((kind == FL_VAL_BF16 && !A->bf16_hi.p) || (kind == FL_VAL_BF16X2 && !A->bf16_pair.p)) {
         set_error("fl_spmv: BF16 values not prepared");
         return FL_ERR_INVALID;
     }
@@ -879,20 +879,20 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
     }
     switch (kind) {
         case FL_VAL_F64:
-            if (win) launch_win<double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, (double *)y, assign, s);
-            else launch_pb<double, double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, A->pb_bins64.p, (double *)y, assign, s);
+            if (win) launch_win<double, double, double, false>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
+            else launch_pb<double, double, double, double, false>(A, A->val64.p, (const double *)x, A->pb_bins64.p, (double *)y, assign, s);
             return FL_OK;
         case FL_VAL_F32:
-            if (win) launch_win<float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, (float *)y, assign, s);
-            else launch_pb<float, float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
+            if (win) launch_win<float, float, float, false>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
+            else launch_pb<float, float, float, float, false>(A, A->val32.p, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16:
-            if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
+            if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
+            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16X2:
-            if (win) launch_win<uint16_t, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, (double *)y, assign, s);
-            else launch_pb<uint16_t, float, float, double, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, A->pb_bins32.p, (double *)y, assign, s);
+            if (win) launch_win<uint32_t, float, double, true>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
+            else launch_pb<uint32_t, float, float, double, true>(A, A->bf16_pair.p, (const float *)x, A->pb_bins32.p, (double *)y, assign, s);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);

@@ -181,6 +181,12 @@ __global__ void k_to_bf16x2(const double *__restrict__ in, uint16_t *__restrict_
     }
 }
 
+__global__ void k_pack_pair(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
+                            uint32_t *__restrict__ pair, int count) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long long)gridDim.x * blockDim.x)
+        pair[i] = ((uint32_t)hi[i] << 16) | lo[i];
+}
+
 __global__ void k_recombine(const float *__restrict__ yh, const float *__restrict__ yl, double *__restrict__ y,
                             int m) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
@@ -206,6 +212,11 @@ void prepare_values(fl_matrix *A, int kind, hipStream_t s) {
         A->bf16_hi.alloc(std::max(nnz, 1));
         A->bf16_lo.alloc(std::max(nnz, 1));
         if (nnz) k_to_bf16x2<<<sgrid(nnz), kBlock, 0, s>>>(A->val64.p, A->bf16_hi.p, A->bf16_lo.p, nnz);
+        FL_LAUNCH_CHECK();
+    }
+    if (kind == FL_VAL_BF16X2 && !A->bf16_pair.p) {
+        A->bf16_pair.alloc(std::max(nnz, 1));
+        if (nnz) k_pack_pair<<<sgrid(nnz), kBlock, 0, s>>>(A->bf16_hi.p, A->bf16_lo.p, A->bf16_pair.p, nnz);
         FL_LAUNCH_CHECK();
     }
     FL_HIP(hipStreamSynchronize(s));
