@@ -170,7 +170,9 @@ int fl_matrix_export(const fl_matrix *A, int *perm, int *sortnnz, int *tile_ptr,
 int fl_permute_x_f64(const fl_matrix *A, const double *d_x, double *d_sortx, void *stream);
 int fl_permute_x_f32(const fl_matrix *A, const float *d_x, float *d_sortx, void *stream);
 
-/* Builds (once) the value copy of the given kind from the FP64 values. */
+/* Builds (once) the value copy of the given kind from the FP64 values. FL_VAL_BF16X2 keeps the
+ * hi and lo arrays (FAST kernel) and the pair packed as one 32-bit word per entry (hi << 16 | lo,
+ * read by the batched schedules). */
 int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
 
 /* SpMV kernels:
