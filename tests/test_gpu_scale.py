@@ -124,3 +124,36 @@ def test_pb_two_chunks_parity(gpu, uniform_42m, bt, R):
     dy0 = DeviceArray.zeros(m, np.float32)
     M.spmv(dsx, dy0, fl.KERNEL_PB, fl.VAL_F32)
     assert np.array_equal(ya, dy0.to_host())
+
+
+def test_pb_two_chunks_graph_replays(gpu, uniform_42m):
+    """fl_spmv_assign with the two-chunk PB (four kernels) captured in a hipGraph: every replay,
+    with eager work between replays, equals the eager result to the last FP32 bit or so (the
+    FP64 LDS sums of a band may add in another order)."""
+    m, n, cp, ri, v, x = uniform_42m
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    M.prepare(fl.VAL_F32)
+    M.prepare_schedule(fl.KERNEL_PB, 128, 16384)
+    s = fl.Stream()
+    dx = DeviceArray.from_host(x.astype(np.float32))
+    dsx = DeviceArray(n, np.float32)
+    M.permute_x(dx, dsx)
+    dy = DeviceArray(m, np.float32)
+
+    def step():
+        M.spmv(dsx, dy, fl.KERNEL_PB, fl.VAL_F32, stream=s, assign=True)
+    step()
+    s.synchronize()
+    eager = dy.to_host()
+    g = fl.Graph(s)
+    with g:
+        step()
+    for rep in range(3):
+        dy.copy_from(DeviceArray.from_host(np.full(m, 7.0, np.float32)))
+        fl.synchronize()
+        g.launch()
+        s.synchronize()
+        got = dy.to_host()
+        assert np.allclose(got, eager, rtol=1e-6, atol=1e-6), rep
+        step()
+        s.synchronize()
