@@ -12,14 +12,17 @@ Schedule space (tile width x reduction strategy x batch/band parameters):
   0 dropin_w32   reference tiles and segmented sums (32 lanes per tile), atomics per segment
   1 dropin_w64   the same algorithm on 64-wide tiles
   2 winr64       WIN on 64-tile batches of tiles grouped by row window (owned rows, no atomic)
-  3-5 win{64,128,256}  batches of 64/128/256 tiles, rows compacted to LDS slots, one atomic/row
+  3-5 win{64,128,256}  batches of 64/128/256 tiles, rows compacted to LDS slots, one flush per
+                 row of a batch (a store for rows no other batch touches, else one atomic)
   6-10 pb*       propagation blocking, batch 128/256 tiles x band 4K/8K/16K rows, no atomics
   11 row         entries re-ordered by row (CSR), lanes own rows, no atomics (fl_spmm, k=1)
 
 Training data come from exhaustive sweeps: every schedule is timed on every matrix of a seeded
-synthetic population (`population_spec`), with the matrix and x resident on the GPU; one step is y = 0
-followed by one SpMV in FP32 (value storage FP32, FP64 LDS sums where the schedule has them).
-Preprocessing time is not part of a step (it is paid once per matrix).
+synthetic population (`population_spec`), with the matrix and x resident on the GPU; one step is
+y = A x in FP32 (`fl_spmv_assign`, or `fl_spmm` for `row`; value storage FP32, FP64 LDS sums
+where the schedule has them). Preprocessing time is not part of a step (it is paid once per
+matrix). On a new matrix the network's probabilities pick the candidates that are timed before
+the fastest is kept (`Selector.choose_refined`, `tuning_candidates`).
 """
 from __future__ import annotations
 
