@@ -65,6 +65,31 @@ def test_spmm_row_parity(gpu, kind, k):
         assert rel_err(Y2.ravel(), (Yo + Y0).ravel(), (S + np.abs(Y0)).ravel()) <= TOL[kind]
 
 
+@pytest.mark.parametrize("kind", [fl.VAL_F64, fl.VAL_F32], ids=["f64", "f32"])
+@pytest.mark.parametrize("k", [2, 8, 32])
+def test_spmm_unaligned_blocks_match_aligned(gpu, kind, k):
+    """X and Y blocks at 4- and 8-byte offsets take the narrower-lane kernels (16-byte lanes need
+    16-byte alignment); every lane width sums each Y element in entry order, so the results are
+    bitwise equal to the aligned run."""
+    from fastload_amd._lib import check, lib
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, 12, 1500, 1700, (1.8, 400))
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    M.prepare_rows()
+    dt = np.float64 if kind == fl.VAL_F64 else np.float32
+    X = np.random.default_rng(k).standard_normal((n, k)).astype(dt)
+    want = run_spmm(M, X, kind, k)
+    sz = np.dtype(dt).itemsize
+    for off in (4, 8) if kind == fl.VAL_F32 else (8,):
+        bx = DeviceArray(n * k * sz + 64, np.uint8)
+        by = DeviceArray(m * k * sz + 64, np.uint8)
+        check(lib.fl_memcpy_h2d(bx.ptr + off, X.ctypes.data, n * k * sz), "fl_memcpy_h2d")
+        check(lib.fl_memset_device(by.ptr, 0, m * k * sz + 64), "fl_memset_device")
+        check(lib.fl_spmm(M.handle, kind, k, bx.ptr + off, by.ptr + off, 0, None), "fl_spmm")
+        fl.synchronize()
+        got = by.to_host()[off:off + m * k * sz].view(dt).astype(np.float64).reshape(m, k)
+        assert np.array_equal(got, want), (kind, k, off)
+
+
 def test_spmm_invalid_k(gpu):
     m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, 1, 50, 50, (2.0, 10))
     M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
