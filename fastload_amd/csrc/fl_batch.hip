@@ -169,13 +169,15 @@ static void launch_row_v(const fl_matrix *A, const T *val, const T *X, T *Y, int
     FL_LAUNCH_CHECK();
 }
 
-// Widest vector that divides k and that X and Y are aligned for, 16 bytes at most (k = 32:
-// 3.14 ms with 16-byte lanes against 5.28 ms with 4-byte lanes and 3.84 ms with 32-byte lanes
-// on the 512-matrix batch).
+// Widest vector that keeps at least 4 lanes per row (k >= 4), is at most 16 bytes and that X
+// and Y are aligned for. On the 512-matrix batch: k = 32 took 3.14 ms with 16-byte lanes
+// against 5.28 ms with 4-byte lanes and 3.84 ms with 32-byte lanes; k = 8 took 1.72 ms with
+// 8-byte lanes (4 per row) against 1.78 ms with 16-byte lanes (2 per row).
 template <typename T, typename ACC, int K>
 static void launch_row(const fl_matrix *A, const T *val, const T *X, T *Y, int accumulate, hipStream_t s) {
     constexpr int VT = 16 / (int)sizeof(T);
-    constexpr int VMAX = VT < K ? VT : K;
+    constexpr int VK = K / 4 > 1 ? K / 4 : 1;
+    constexpr int VMAX = VT < VK ? VT : VK;
     const uintptr_t mis = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
     if constexpr (VMAX >= 4) {
         if ((mis & 15) == 0) return launch_row_v<T, ACC, K, 4>(A, val, X, Y, accumulate, s);
