@@ -14,7 +14,7 @@ Variants (schedules x value storage):
            (a store when no other batch touches the row, else one atomic)
   winr*    the same kernel on tiles grouped by row window (most rows owned by one batch)
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
-  row_*    entries re-ordered by row, lanes own rows, no atomics
+  row_*    entries re-ordered by row, four lanes per row, no atomics
   learned_* the schedule the learned selector picks for the matrix: features and MLP inference
            on the GPU (committed trained weights) rank the 12 schedules, those of probability
            >= 5 % (at most 5) are timed on the matrix and the fastest is used; in each value

@@ -15,7 +15,7 @@ Schedule space (tile width x reduction strategy x batch/band parameters):
   3-5 win{64,128,256}  batches of 64/128/256 tiles, rows compacted to LDS slots, one flush per
                  row of a batch (a store for rows no other batch touches, else one atomic)
   6-10 pb*       propagation blocking, batch 128/256 tiles x band 4K/8K/16K rows, no atomics
-  11 row         entries re-ordered by row (CSR), lanes own rows, no atomics (fl_spmm, k=1)
+  11 row         entries re-ordered by row (CSR), four lanes per row, no atomics (fl_spmm, k=1)
 
 Training data come from exhaustive sweeps: every schedule is timed on every matrix of a seeded
 synthetic population (`population_spec`), with the matrix and x resident on the GPU; one step is

@@ -224,8 +224,10 @@ int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int ba
  * Y (m x k, row-major) = A * X  (accumulate = 0)  or  Y += A * X  (accumulate = 1), where X is
  * n x k, row-major, in the ORIGINAL column order (no permutation). k in {1,2,4,8,16,32,64};
  * val_kind FL_VAL_F64 (double X/Y, FP64 sums) or FL_VAL_F32 (float X/Y, FP32 sums).
- * The entries are re-ordered by row once (fl_matrix_prepare_rows); k lanes own a row, X rows
- * are gathered whole and each Y row is written once, without atomics. */
+ * The entries are re-ordered by row once (fl_matrix_prepare_rows); a few lanes own a row (k = 1:
+ * four lanes, each summing every fourth entry; k >= 2: each lane gathers up to 16 bytes of an X
+ * row, 16-byte lanes need X and Y 16-byte aligned), and each Y row is written once, without
+ * atomics. Results do not depend on the lane width or on the other rows of the matrix. */
 int fl_matrix_prepare_rows(fl_matrix *A, void *stream);
 int fl_spmm(fl_matrix *A, int val_kind, int k, const void *d_X, void *d_Y, int accumulate, void *stream);
 
