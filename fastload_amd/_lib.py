@@ -37,6 +37,8 @@ SIGNATURES = {
     "fl_formatTransform": (C.c_int, [f64p, f64p, C.POINTER(SlideMatrix), vp, vp, vp, C.c_int, C.c_int, C.c_int, i32p]),
     "fl_FastLoad_spmv": (C.c_int, [C.c_char_p, C.POINTER(SlideMatrix), C.c_int, C.c_int, C.c_int, vp, vp, vp, f64p]),
     "fl_Tile_destroy": (None, [C.POINTER(SlideMatrix)]),
+    "fl_FastLoad_cpu": (C.c_int, [C.POINTER(SlideMatrix), C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, i32p]),
+    "fl_FastLoad_cpu_f32": (C.c_int, [C.POINTER(SlideMatrix), vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int]),
     "fl_device_count": (C.c_int, [i32p]),
     "fl_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
     "fl_free_device": (C.c_int, [vp]),
@@ -105,9 +107,12 @@ lib = _load()
 
 
 class FastLoadError(RuntimeError):
-    def __init__(self, rc: int, where: str):
-        msg = lib.fl_last_error().decode(errors="replace")
-        super().__init__(f"{where} failed with status {rc}: {msg}")
+    """A library call failed (message from fl_last_error), or a wrapper refused its arguments
+    before calling into the library (rc -8, FL_ERR_INVALID, with the wrapper's own message)."""
+
+    def __init__(self, rc: int, where: str, from_library: bool = True):
+        msg = lib.fl_last_error().decode(errors="replace") if from_library else ""
+        super().__init__(f"{where} failed with status {rc}: {msg}" if from_library else f"{where} (status {rc})")
         self.rc = rc
 
 

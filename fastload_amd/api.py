@@ -141,7 +141,53 @@ def FastLoad_spmv(tiles: Tiles, nnz: int, colA: int, rowA: int, x, y_golden=None
     return y[:rowA], t.value, rc == 0
 
 
+def FastLoad_cpu(tiles: Tiles, nnz: int, rowA: int, colA: int, x, y_golden=None, threads: int = 1, y=None):
+    """src/FastLoad_CPU.h:2-46 on the host -> (y, errors). x is sortx (colA entries); y (rowA
+    entries, zeros unless given) accumulates the product. threads = 1 is the reference's serial
+    loop (bit-identical y); 0 = every available core, > 1 that many (atomic adds, so the result
+    equals the serial one up to the order of additions). With y_golden, errors counts the rows
+    differing from it exactly (the reference's check) and a mismatch prints "Check NO PASS!"."""
+    sm, keep = _slide(tiles)
+    xx = np.ascontiguousarray(x, np.float64)
+    yy = np.zeros(max(rowA, 1)) if y is None else y
+    if yy.dtype != np.float64 or not yy.flags.c_contiguous or yy.size < rowA:
+        raise ValueError("FastLoad_cpu: y must be a contiguous float64 array of rowA entries")
+    if xx.size < colA:
+        raise ValueError("FastLoad_cpu: x needs colA entries")
+    g = None if y_golden is None else np.ascontiguousarray(y_golden, np.float64)
+    if g is not None and g.size < rowA:
+        raise ValueError("FastLoad_cpu: y_golden needs rowA entries")
+    err = C.c_int()
+    check(lib.fl_FastLoad_cpu(C.byref(sm), nnz, rowA, colA, _p(xx) if xx.size else _p(np.zeros(1)), _p(yy), _p(g),
+                              threads, C.byref(err)), "FastLoad_cpu")
+    del keep
+    return yy[:rowA], err.value
+
+
+def FastLoad_cpu_f32(tiles: Tiles, sortval32, nnz: int, rowA: int, colA: int, x32, threads: int = 1):
+    """The FastLoad_cpu loop in FP32 (values sortval32 in tile order) -> y (float32)."""
+    sm, keep = _slide(tiles)
+    v = np.ascontiguousarray(sortval32, np.float32)
+    xx = np.ascontiguousarray(x32, np.float32)
+    if v.size < nnz or xx.size < colA:
+        raise ValueError("FastLoad_cpu_f32: sortval needs nnz entries and x colA entries")
+    y = np.zeros(max(rowA, 1), np.float32)
+    check(lib.fl_FastLoad_cpu_f32(C.byref(sm), _p(v) if v.size else _p(np.zeros(1, np.float32)), nnz, rowA, colA,
+                                  _p(xx) if xx.size else _p(np.zeros(1, np.float32)), _p(y), threads),
+          "FastLoad_cpu_f32")
+    del keep
+    return y[:rowA]
+
+
 # ---------------------------------------------------------------- device-resident matrix
+def _need(d: DeviceArray, count: int, dtype, what: str):
+    """Refuses a device buffer too small or of the wrong element type before any kernel sees it."""
+    if np.dtype(d.dtype) != np.dtype(dtype):
+        raise FastLoadError(-8, f"{what}: expected {np.dtype(dtype).name}, got {np.dtype(d.dtype).name}", from_library=False)
+    if d.count < count:
+        raise FastLoadError(-8, f"{what}: needs {count} elements, buffer has {d.count}", from_library=False)
+
+
 class FastLoadMatrix:
     """ColSort + formatTransform run on the GPU and kept resident (include/fastload.h)."""
 
@@ -197,16 +243,31 @@ class FastLoadMatrix:
 
     def spmm(self, dX: DeviceArray, dY: DeviceArray, k: int, val_kind=VAL_F32, accumulate=False, stream=None):
         """Y = A X (or Y += A X); X is n x k and Y is m x k, row-major, original column order."""
+        if val_kind not in (VAL_F64, VAL_F32):
+            raise FastLoadError(-8, "spmm: val_kind must be VAL_F64 or VAL_F32", from_library=False)
+        dt = np.float64 if val_kind == VAL_F64 else np.float32
+        _need(dX, self.n * k, dt, "spmm X")
+        _need(dY, self.m * k, dt, "spmm Y")
         check(lib.fl_spmm(self.handle, val_kind, k, dX.ptr, dY.ptr, 1 if accumulate else 0,
                           stream.ptr if stream is not None else None), "fl_spmm")
 
     def permute_x(self, dx: DeviceArray, dsx: DeviceArray, stream=None):
+        if dx.dtype not in (np.float64, np.float32):
+            raise FastLoadError(-8, "permute_x: x must be float64 or float32", from_library=False)
+        _need(dx, self.n, dx.dtype, "permute_x x")
+        _need(dsx, self.n, dx.dtype, "permute_x sortx")
         fn = lib.fl_permute_x_f64 if dx.dtype == np.float64 else lib.fl_permute_x_f32
         check(fn(self.handle, dx.ptr, dsx.ptr, stream.ptr if stream is not None else None), "fl_permute_x")
 
     def spmv(self, dsx: DeviceArray, dy: DeviceArray, kernel=KERNEL_DROPIN, val_kind=VAL_F64, work=None, stream=None,
              assign=False):
         """y += A * sortx on the device (y = A * sortx with assign=True)."""
+        if val_kind not in (VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2):
+            raise FastLoadError(-8, f"spmv: unknown val_kind {val_kind}", from_library=False)
+        _need(dsx, self.n, np.float64 if val_kind == VAL_F64 else np.float32, "spmv sortx")
+        _need(dy, self.m, np.float64 if val_kind in (VAL_F64, VAL_BF16X2) else np.float32, "spmv y")
+        if work is not None:
+            _need(work, 2 * self.m, np.float32, "spmv work")
         fn = lib.fl_spmv_assign if assign else lib.fl_spmv
         check(fn(self.handle, kernel, val_kind, dsx.ptr, dy.ptr, work.ptr if work is not None else None,
                  stream.ptr if stream is not None else None), "fl_spmv_assign" if assign else "fl_spmv")

@@ -24,6 +24,8 @@ CXX = shutil.which("g++") or "g++"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-march=x86-64-v3"]
+# per-file additions: the host SpMV must round like the reference's host build (no FMA contraction)
+FILE_FLAGS = {"fl_cpu.cpp": ["-ffp-contract=off", "-pthread"]}
 
 
 def _run(cmd):
@@ -39,7 +41,7 @@ def _compile(src: Path) -> Path:
     if src.suffix == ".hip":
         cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
     else:
-        cmd = [CXX, *CXX_FLAGS, "-c", str(src), "-o", str(obj)]
+        cmd = [CXX, *CXX_FLAGS, *FILE_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     _run(cmd)
     return obj
 

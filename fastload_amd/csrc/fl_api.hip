@@ -1,5 +1,6 @@
 // extern "C" entry points of libfastload.so (declared in include/fastload.h).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -277,23 +278,18 @@ int fl_FastLoad_spmv(const char *filename, const slide_matrix *M, int nnz, int c
     FL_HIP(hipMemsetAsync(dy.p, 0, sizeof(double) * std::max(rowA, 1), s));
     int rc = spmv(&A, FL_KERNEL_DROPIN, FL_VAL_F64, dx.p, dy.p, nullptr, false, s);
     if (rc) return rc;
-    hipEvent_t e0, e1;
-    FL_HIP(hipEventCreate(&e0));
-    FL_HIP(hipEventCreate(&e1));
+    // FastLoad_GPU.h:349-374: 100 x (clear y, then the host clock around launch + synchronize)
     double total = 0.0;
-    const int reps = 100;  // FastLoad_GPU.h:349
+    const int reps = 100;
     for (int r = 0; r < reps; ++r) {
         FL_HIP(hipMemsetAsync(dy.p, 0, sizeof(double) * std::max(rowA, 1), s));
-        FL_HIP(hipEventRecord(e0, s));
+        FL_HIP(hipStreamSynchronize(s));
+        const auto t0 = std::chrono::steady_clock::now();
         rc = spmv(&A, FL_KERNEL_DROPIN, FL_VAL_F64, dx.p, dy.p, nullptr, false, s);
-        FL_HIP(hipEventRecord(e1, s));
-        FL_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        FL_HIP(hipEventElapsedTime(&ms, e0, e1));
-        total += ms;
+        FL_HIP(hipStreamSynchronize(s));
+        total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (rc) return rc;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     const double mean = total / reps;
     download(y, dy.p, rowA, s);
     FL_HIP(hipStreamSynchronize(s));

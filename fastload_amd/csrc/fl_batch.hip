@@ -359,6 +359,19 @@ int fl_batch_assemble(int count, const int *m, const int *n, const int *const *c
             set_error("fl_batch_assemble: matrix %d invalid", i);
             return FL_ERR_INVALID;
         }
+        // colptr[i] must be a column pointer (starts at 0, never decreases) before its last entry
+        // sizes the output and its ranges are copied
+        const int *cpi = colptr[i];
+        bool ok = cpi[0] == 0;
+        for (int c = 0; ok && c < n[i]; ++c) ok = cpi[c] <= cpi[c + 1];
+        if (!ok) {
+            set_error("fl_batch_assemble: matrix %d colptr is not a column pointer (colptr[0] != 0 or decreasing)", i);
+            return FL_ERR_INVALID;
+        }
+        if (cpi[n[i]] > 0 && (!rowidx[i] || !val[i])) {
+            set_error("fl_batch_assemble: matrix %d has entries but NULL rowidx/val", i);
+            return FL_ERR_INVALID;
+        }
         MM += m[i];
         NN += n[i];
         NZ += colptr[i][n[i]];

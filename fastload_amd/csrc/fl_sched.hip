@@ -861,18 +861,9 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
     }
 }
 
+// The caller (spmv) has checked that the schedule and the value copy are prepared.
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s) {
     const bool win = (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR);
-    if (win && (!A->win_ready || A->ord_kind != (kernel == FL_KERNEL_WINR ? 1 : 0))) {
-        set_error("fl_spmv: %s schedule not prepared", kernel == FL_KERNEL_WINR ? "WINR" : "WIN");
-        return FL_ERR_INVALID;
-    }
-    if (!win && !A->pb_ready) { set_error("fl_spmv: PB schedule not prepared"); return FL_ERR_INVALID; }
-    if (kind == FL_VAL_F32 && !A->val32.p) { set_error("fl_spmv: FP32 values not prepared"); return FL_ERR_INVALID; }
-    if ((kind == FL_VAL_BF16 && !A->bf16_hi.p) || (kind == FL_VAL_BF16X2 && !A->bf16_pair.p)) {
-        set_error("fl_spmv: BF16 values not prepared");
-        return FL_ERR_INVALID;
-    }
     if (!win) {
         if (kind == FL_VAL_F64) { if (!A->pb_bins64.p) A->pb_bins64.alloc(std::max(A->nnz, 1)); }
         else if (!A->pb_bins32.p) A->pb_bins32.alloc(std::max(A->nnz, 1));

@@ -258,50 +258,84 @@ static void launch_fast(const fl_matrix *A, const V *val, const V *val_lo, const
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s);
 void zero_fill(void *d, size_t bytes, hipStream_t s);
 
+static int refuse(const char *msg) {
+    set_error("%s", msg);
+    return FL_ERR_INVALID;
+}
+
+// Every condition under which a call would be refused, checked before anything is enqueued, so
+// that an invalid call leaves y as it was.
+static int spmv_check(const fl_matrix *A, int kernel, int kind, const void *work) {
+    if (kind < FL_VAL_F64 || kind > FL_VAL_BF16X2) {
+        set_error("fl_spmv: unknown value kind %d", kind);
+        return FL_ERR_INVALID;
+    }
+    if (kernel < FL_KERNEL_DROPIN || kernel > FL_KERNEL_WINR) {
+        set_error("fl_spmv: unknown kernel %d", kernel);
+        return FL_ERR_INVALID;
+    }
+    if (A->tilenum == 0) return FL_OK;  // nothing to multiply: y = 0 (assign) or unchanged
+    const bool f32_missing = kind == FL_VAL_F32 && !A->val32.p;
+    if (kernel == FL_KERNEL_DROPIN) {
+        if (kind != FL_VAL_F64 && kind != FL_VAL_F32)
+            return refuse("fl_spmv: the drop-in kernel supports FL_VAL_F64 and FL_VAL_F32");
+        if (f32_missing) return refuse("fl_spmv: FP32 values not prepared");
+        return FL_OK;
+    }
+    if (A->W != 64) {
+        return refuse(kernel == FL_KERNEL_FAST ? "fl_spmv: the FAST kernel needs a 64-wide tile layout"
+                                               : "fl_spmv: batched schedules need a 64-wide tile layout");
+    }
+    if (f32_missing) return refuse("fl_spmv: FP32 values not prepared");
+    if (kernel == FL_KERNEL_FAST) {
+        if ((kind == FL_VAL_BF16 || kind == FL_VAL_BF16X2) && !A->bf16_hi.p)
+            return refuse("fl_spmv: BF16 values not prepared");
+        if (kind == FL_VAL_BF16X2 && !work) return refuse("fl_spmv: FL_VAL_BF16X2 needs a work buffer of 2*m floats");
+        return FL_OK;
+    }
+    if (kernel == FL_KERNEL_PB) {
+        if (!A->pb_ready) return refuse("fl_spmv: PB schedule not prepared");
+    } else if (!A->win_ready || A->ord_kind != (kernel == FL_KERNEL_WINR ? 1 : 0)) {
+        return refuse(kernel == FL_KERNEL_WINR ? "fl_spmv: WINR schedule not prepared"
+                                               : "fl_spmv: WIN schedule not prepared");
+    }
+    if ((kind == FL_VAL_BF16 && !A->bf16_hi.p) || (kind == FL_VAL_BF16X2 && !A->bf16_pair.p))
+        return refuse("fl_spmv: BF16 values not prepared");
+    return FL_OK;
+}
+
 // assign: y = A x instead of y += A x. The PB schedule writes every row once and needs no
 // clearing; the other kernels accumulate, so y is cleared with a fill kernel first (WIN and
 // WINR then store the rows a single batch owns and add the shared ones atomically).
 int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s) {
-    if (kind < FL_VAL_F64 || kind > FL_VAL_BF16X2) { set_error("fl_spmv: unknown value kind %d", kind); return FL_ERR_INVALID; }
+    const int rc = spmv_check(A, kernel, kind, work);
+    if (rc) return rc;
     const size_t ybytes = (size_t)A->m * ((kind == FL_VAL_F64 || kind == FL_VAL_BF16X2) ? 8 : 4);
     if (assign && (A->tilenum == 0 || kernel != FL_KERNEL_PB)) zero_fill(y, ybytes, s);
     if (A->tilenum == 0) return FL_OK;
-    if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR || kernel == FL_KERNEL_PB) {
-        if (A->W != 64) { set_error("fl_spmv: batched schedules need a 64-wide tile layout"); return FL_ERR_INVALID; }
+    if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR || kernel == FL_KERNEL_PB)
         return spmv_batched(A, kernel, kind, x, y, assign, s);
-    }
     if (kernel == FL_KERNEL_DROPIN) {
         if (kind == FL_VAL_F64) {
             if (A->W == 32) launch_dropin<double, 32>(A, (const double *)x, (double *)y, s);
             else launch_dropin<double, 64>(A, (const double *)x, (double *)y, s);
-            return FL_OK;
-        }
-        if (kind == FL_VAL_F32) {
-            if (!A->val32.p) { set_error("fl_spmv: FP32 values not prepared"); return FL_ERR_INVALID; }
+        } else {
             if (A->W == 32) launch_dropin<float, 32>(A, (const float *)x, (float *)y, s);
             else launch_dropin<float, 64>(A, (const float *)x, (float *)y, s);
-            return FL_OK;
         }
-        set_error("fl_spmv: the drop-in kernel supports FL_VAL_F64 and FL_VAL_F32");
-        return FL_ERR_INVALID;
+        return FL_OK;
     }
-    if (kernel != FL_KERNEL_FAST) { set_error("fl_spmv: unknown kernel %d", kernel); return FL_ERR_INVALID; }
-    if (A->W != 64) { set_error("fl_spmv: the FAST kernel needs a 64-wide tile layout"); return FL_ERR_INVALID; }
-    switch (kind) {
+    switch (kind) {  // FAST
         case FL_VAL_F64:
             launch_fast<double, double, double, false>(A, A->val64.p, nullptr, (const double *)x, (double *)y, nullptr, s);
             return FL_OK;
         case FL_VAL_F32:
-            if (!A->val32.p) { set_error("fl_spmv: FP32 values not prepared"); return FL_ERR_INVALID; }
             launch_fast<float, float, float, false>(A, A->val32.p, nullptr, (const float *)x, (float *)y, nullptr, s);
             return FL_OK;
         case FL_VAL_BF16:
-            if (!A->bf16_hi.p) { set_error("fl_spmv: BF16 values not prepared"); return FL_ERR_INVALID; }
             launch_fast<uint16_t, float, float, false>(A, A->bf16_hi.p, nullptr, (const float *)x, (float *)y, nullptr, s);
             return FL_OK;
-        case FL_VAL_BF16X2: {
-            if (!A->bf16_hi.p) { set_error("fl_spmv: BF16 values not prepared"); return FL_ERR_INVALID; }
-            if (!work) { set_error("fl_spmv: FL_VAL_BF16X2 needs a work buffer of 2*m floats"); return FL_ERR_INVALID; }
+        default: {  // FL_VAL_BF16X2
             float *yh = (float *)work, *yl = yh + A->m;
             zero_fill(yh, sizeof(float) * 2 * (size_t)A->m, s);
             launch_fast<uint16_t, float, float, true>(A, A->bf16_hi.p, A->bf16_lo.p, (const float *)x, yh, yl, s);
@@ -309,9 +343,6 @@ int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work,
             FL_LAUNCH_CHECK();
             return FL_OK;
         }
-        default:
-            set_error("fl_spmv: unknown value kind %d", kind);
-            return FL_ERR_INVALID;
     }
 }
 

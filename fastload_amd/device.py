@@ -49,12 +49,26 @@ class DeviceArray:
             check(lib.fl_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "fl_memcpy_d2h")
         return out
 
+    def view(self, offset: int, count: int) -> "DeviceArray":
+        """Elements [offset, offset + count) of this allocation, without a copy. The view does
+        not own memory: keep the parent alive while the view is used."""
+        if offset < 0 or count < 0 or offset + count > self.count:
+            raise ValueError(f"view [{offset}, {offset + count}) outside {self.count} elements")
+        v = object.__new__(DeviceArray)
+        v.dtype, v.count, v.nbytes = self.dtype, int(count), self.dtype.itemsize * int(count)
+        v.ptr = self.ptr + self.dtype.itemsize * int(offset)
+        v._parent = self
+        return v
+
     def copy_from(self, other: "DeviceArray"):
         assert other.nbytes == self.nbytes
         if self.nbytes:
             check(lib.fl_memcpy_d2d(self.ptr, other.ptr, self.nbytes), "fl_memcpy_d2d")
 
     def free(self):
+        if getattr(self, "_parent", None) is not None:  # a view: the parent owns the memory
+            self.ptr = None
+            return
         if getattr(self, "ptr", None):
             lib.fl_free_device(self.ptr)
             self.ptr = None

@@ -63,7 +63,8 @@ typedef struct {
 
 /* ------------------------------------------------- reference host API (host pointers)
  * Each call uploads its inputs, computes on the GPU, and downloads its outputs, as the
- * reference does. Times are GPU-event times of the device work in milliseconds.
+ * reference does. Times are in milliseconds: GPU-event times of the device work, except for
+ * fl_FastLoad_spmv, which measures as the reference does (see there).
  */
 
 /* Replaces mmio_allinone (src/auxiliary.h:65-232): Matrix Market file -> 0-based CSR with
@@ -98,10 +99,29 @@ int fl_formatTransform(double *timeFormatTran, double *timeFormatclas, slide_mat
  * x is the permuted vector (sortx, colA entries); y (rowA entries, original row order)
  * receives A*x. As in the reference: one dry run, then 100 timed launches (y zeroed before
  * each), a 1%-relative check of y against y_golden (skipped if y_golden is NULL) printed as
- * PASS/NO PASS. *time_ms (if not NULL) receives the mean kernel time (GPU events). Returns
+ * PASS/NO PASS. *time_ms (if not NULL) receives the mean time of one launch, measured as the
+ * reference does (host clock around launch + synchronize, src/FastLoad_GPU.h:349-374). Returns
  * FL_OK, or 1 when the check fails. filename is only used in the printout (may be NULL). */
 int fl_FastLoad_spmv(const char *filename, const slide_matrix *matrix, int nnz, int colA, int rowA, const double *x,
                      double *y, const double *y_golden, double *time_ms);
+
+/* Replaces FastLoad_cpu (src/FastLoad_CPU.h:2-46; called at src/main.cu:267): the host loop over
+ * the tiles in tile order, y[sortrowidx[j]] += sortval[j] * x[tile_colidx[t] + (j - tile_ptr[t]) %
+ * tile_len[t]] with x = sortx (colA entries); y (rowA entries) accumulates, the caller zeroes it.
+ * threads = 1 runs the reference's serial loop with its roundings (bit-identical y); threads = 0
+ * uses every core the process may run on, threads > 1 that many, splitting the tiles into
+ * contiguous ranges and adding into y atomically (same sum up to the order of additions).
+ * When y_goldencsc is not NULL, y is compared with it for exact equality as the reference does
+ * (src/FastLoad_CPU.h:33-45; meaningful on integer-valued data): mismatches are counted into
+ * *errors (if not NULL), printed as "Check NO PASS! error_count_check_cpu= N", and the call
+ * returns 1. Returns FL_OK, FL_ERR_INVALID (bad layout), or FL_ERR_RANGE (a row index outside
+ * [0, rowA); those elements are skipped). */
+int fl_FastLoad_cpu(const slide_matrix *matrix, int nnz, int rowA, int colA, const double *x, double *y,
+                    const double *y_goldencsc, int threads, int *errors);
+/* The same loop in FP32 (no reference counterpart; the reference is FP64 only): values sortval
+ * [nnz] in tile order, x and y float, products and sums rounded to FP32 in the same order. */
+int fl_FastLoad_cpu_f32(const slide_matrix *matrix, const float *sortval, int nnz, int rowA, int colA,
+                        const float *x, float *y, int threads);
 
 /* Replaces Tile_destroy (src/format.h:29-42): frees the arrays fl_formatTransform set
  * (NULL fields are skipped) and zeroes the struct. */

@@ -55,6 +55,13 @@ inline void FastLoad_spmv(char *filename, slide_matrix *matrix, int nnz, int col
                                    "FastLoad_spmv");
 }
 
+// src/FastLoad_CPU.h:2-9 (the reference's serial host loop: one thread, same roundings)
+inline void FastLoad_cpu(slide_matrix *matrix, int nnz, int rowA, int colA, double *x, double *y,
+                         double *y_goldencsc) {
+    fastload_compat_detail::report(fl_FastLoad_cpu(matrix, nnz, rowA, colA, x, y, y_goldencsc, 1, nullptr),
+                                   "FastLoad_cpu");
+}
+
 // src/format.h:29-42
 inline void Tile_destroy(slide_matrix *matrix) { fl_Tile_destroy(matrix); }
 

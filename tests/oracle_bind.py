@@ -67,6 +67,7 @@ def lib():
         L.or_spmv_oracle.argtypes = [C.c_int, C.c_int, _i32p, _i32p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, _f64p, _f64p]
         L.or_bf16_split.argtypes = [C.c_int, _f32p, _u16p, C.c_void_p]
+        L.or_exclusive_scan.argtypes = [_i32p, C.c_int]
         _lib = L
     return _lib
 
@@ -150,6 +151,13 @@ def pipeline(m, n, colptr, rowidx, val, x, W=32):
     perm, sortnnz, tmp_row, tmp_val, sortx = colsort(n, colptr, rowidx, val, x)
     out = transform(n, nnz, sortnnz, tmp_row, tmp_val, W)
     out.update(perm=perm, sortnnz=sortnnz, tmp_row=tmp_row, tmp_val=tmp_val, sortx=sortx, m=m, n=n, nnz=nnz, W=W)
+    return out
+
+
+def exclusive_scan(a):
+    """AUX:29-44: in-place exclusive prefix sum (returns a scanned copy)."""
+    out = np.array(a, dtype=np.int32, copy=True)
+    lib().or_exclusive_scan(out if out.size else np.zeros(1, np.int32), int(out.size))
     return out
 
 

@@ -198,6 +198,7 @@ def test_reference_style_driver(gpu, tmp_path):
     r = subprocess.run([str(exe), str(GOLDEN / "1138_bus.mtx")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "Check FASTLOAD GPU PASS!" in r.stdout, r.stdout + r.stderr
+    assert "Check NO PASS" not in r.stdout, r.stdout  # FastLoad_cpu and FastLoad_spmv both exact
 
 
 @pytest.mark.parametrize("kernel,kind,W", [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_FAST, fl.VAL_F32, 64),
@@ -284,3 +285,26 @@ def test_random_shapes_layout_and_schedules(gpu, seed):
             R = int(rng.choice([1024, 4096, 16384]))
             y = device_y(M, x, kernel, kind, bt=bt, R=R)
             assert rel_err(y, y_or, scale) <= TOL[kind], (seed, m, n, W, kernel, kind, bt, R)
+
+
+@pytest.mark.parametrize("case", ["win_unprepared", "pb_unprepared", "winr_unprepared", "dropin_bf16", "fast_w32",
+                                  "f32_values_missing", "unknown_kernel"])
+def test_refused_assign_leaves_y_unchanged(gpu, case):
+    """A call the library refuses returns FL_ERR_INVALID before anything is enqueued: y keeps
+    what it held (fl_spmv_assign clears y only once the call is known to be valid)."""
+    from fastload_amd import DeviceArray
+    _, m, n, cp, ri, v, x = next(c for c in CASES if c[0] == "zipf")
+    W = 32 if case == "fast_w32" else 64
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=W)
+    if case != "f32_values_missing":
+        M.prepare(fl.VAL_F32)
+    kernel, kind = {"win_unprepared": (fl.KERNEL_WIN, fl.VAL_F32), "pb_unprepared": (fl.KERNEL_PB, fl.VAL_F32),
+                    "winr_unprepared": (fl.KERNEL_WINR, fl.VAL_F32), "dropin_bf16": (fl.KERNEL_DROPIN, fl.VAL_BF16),
+                    "fast_w32": (fl.KERNEL_FAST, fl.VAL_F32), "f32_values_missing": (fl.KERNEL_DROPIN, fl.VAL_F32),
+                    "unknown_kernel": (9, fl.VAL_F32)}[case]
+    dsx = DeviceArray.from_host(np.asarray(x, np.float32))
+    dy = DeviceArray.from_host(np.full(m, 123.0, np.float32))
+    with pytest.raises(fl.FastLoadError) as e:
+        M.spmv(dsx, dy, kernel, kind, assign=True)
+    assert e.value.rc == -8
+    assert np.all(dy.to_host() == 123.0)

@@ -201,3 +201,125 @@ def test_batch_assemble_block_diagonal():
     assert co.tolist() == list(np.cumsum([0] + [m[1] for m in mats]))
     with pytest.raises(fl.FastLoadError):  # row index outside its block
         fl.batch_assemble([(2, 1, np.array([0, 1], np.int32), np.array([5], np.int32), np.ones(1))])
+
+
+# ------------------------------------------------------------- FastLoad_cpu (product, host)
+def _golden_tiles(g, W):
+    return fl.Tiles(int(g[f"w{W}_tilenum"]), int(g[f"w{W}_segsum"]), g[f"w{W}_tile_ptr"], g[f"w{W}_tile_len"],
+                    g[f"w{W}_tile_colidx"], g[f"w{W}_tile_format"], g[f"w{W}_sortrowidx"], g[f"w{W}_sortval"],
+                    g[f"w{W}_sortrowindex"], g[f"w{W}_countrow"], g[f"w{W}_segmentoffset"])
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_fastload_cpu_serial_bitwise_vs_reference(W):
+    """fl_FastLoad_cpu with one thread is the reference loop (src/FastLoad_CPU.h:19-31) with its
+    roundings: on float data its y equals the reference's own FastLoad_cpu output (fixture written
+    from the reference source) bit for bit."""
+    g = np.load(GOLDEN / "ref_layout_symgeom.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    y, errors = fl.FastLoad_cpu(_golden_tiles(g, W), nnz, m, n, g["sortx"], threads=1)
+    assert errors == 0
+    assert np.array_equal(y.view(np.uint64), g[f"w{W}_y"].view(np.uint64))
+
+
+@pytest.mark.parametrize("threads", [0, 2, 3, 8])
+def test_fastload_cpu_threads(threads, capfd):
+    """Several threads: exact on integer data (the reference's own check passes), within FP64
+    round-off of the serial result on float data."""
+    g = np.load(GOLDEN / "ref_layout_1138_bus.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    y, errors = fl.FastLoad_cpu(_golden_tiles(g, 32), nnz, m, n, g["sortx"], y_golden=g["w32_y"], threads=threads)
+    assert errors == 0 and np.array_equal(y, g["w32_y"])
+    assert "NO PASS" not in capfd.readouterr().out
+    s = np.load(GOLDEN / "ref_layout_symgeom.npz")
+    ms, ns, nzs = int(s["m"]), int(s["n"]), int(s["nnz"])
+    y2, _ = fl.FastLoad_cpu(_golden_tiles(s, 32), nzs, ms, ns, s["sortx"], threads=threads)
+    scale = np.abs(s["w32_y"]).max()
+    assert np.abs(y2 - s["w32_y"]).max() <= 1e-13 * scale
+
+
+def test_fastload_cpu_golden_check_counts_mismatches(capfd):
+    """src/FastLoad_CPU.h:33-45: exact comparison with y_goldencsc, mismatches counted and
+    printed; the call then returns 1 (not an error)."""
+    g = np.load(GOLDEN / "ref_layout_1138_bus.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    bad = g["w32_y"].copy()
+    bad[[3, 77, 500]] += 1.0
+    y, errors = fl.FastLoad_cpu(_golden_tiles(g, 32), nnz, m, n, g["sortx"], y_golden=bad)
+    assert errors == 3
+    assert "Check NO PASS! error_count_check_cpu= 3" in capfd.readouterr().out
+
+
+def test_fastload_cpu_f32_matches_oracle():
+    g = np.load(GOLDEN / "ref_layout_symgeom.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    t = _golden_tiles(g, 64)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(nnz).astype(np.float32)
+    x = rng.standard_normal(n).astype(np.float32)
+    want = ob.fastload_cpu_f32(dict(tilenum=t.tilenum, tile_ptr=t.tile_ptr, tile_len=t.tile_len,
+                                    tile_colidx=t.tile_colidx, sortrowidx=t.sortrowidx), v, x, m)
+    got = fl.FastLoad_cpu_f32(t, v, nnz, m, n, x, threads=1)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    got4 = fl.FastLoad_cpu_f32(t, v, nnz, m, n, x, threads=4)
+    assert np.abs(got4 - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_fastload_cpu_rejects_bad_layouts():
+    g = np.load(GOLDEN / "ref_layout_1138_bus.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    t = _golden_tiles(g, 32)
+    bad = fl.Tiles(**{**t.__dict__, "tile_len": t.tile_len + 1})  # element counts no longer match lengths
+    with pytest.raises(fl.FastLoadError) as e:
+        fl.FastLoad_cpu(bad, nnz, m, n, g["sortx"])
+    assert e.value.rc == -8
+    with pytest.raises(fl.FastLoadError) as e:  # columns beyond colA
+        fl.FastLoad_cpu(t, nnz, m, n - 5, g["sortx"])
+    assert e.value.rc == -8
+    rows = t.sortrowidx.copy()
+    rows[10] = m + 3
+    with pytest.raises(fl.FastLoadError) as e:
+        fl.FastLoad_cpu(fl.Tiles(**{**t.__dict__, "sortrowidx": rows}), nnz, m, n, g["sortx"], threads=2)
+    assert e.value.rc == -5
+
+
+# ------------------------------------------------------------------- input validation (host)
+def test_batch_assemble_rejects_bad_colptr():
+    good = (3, 3, np.array([0, 1, 2, 3], np.int32), np.array([0, 1, 2], np.int32), np.ones(3))
+    for cp in ([0, 10, 0, 3], [1, 1, 2, 3], [0, 2, 1, 3]):
+        bad = (3, 3, np.array(cp, np.int32), np.array([0, 1, 2], np.int32), np.ones(3))
+        with pytest.raises(fl.FastLoadError) as e:
+            fl.batch_assemble([good, bad])
+        assert e.value.rc == -8, cp
+
+
+def _fake_matrix(m, n):
+    M = object.__new__(fl.FastLoadMatrix)  # no device: only the host-side checks run
+    M.m, M.n, M.handle = m, n, None
+    return M
+
+
+def _buf(count, dtype):
+    from types import SimpleNamespace
+    return SimpleNamespace(count=count, dtype=np.dtype(dtype), ptr=None, nbytes=count * np.dtype(dtype).itemsize)
+
+
+@pytest.mark.parametrize("call", [
+    lambda M: M.spmv(_buf(9, np.float32), _buf(7, np.float32), fl.KERNEL_PB, fl.VAL_F32),   # x too short
+    lambda M: M.spmv(_buf(10, np.float32), _buf(6, np.float32), fl.KERNEL_PB, fl.VAL_F32),  # y too short
+    lambda M: M.spmv(_buf(10, np.float64), _buf(7, np.float32), fl.KERNEL_PB, fl.VAL_F32),  # x dtype
+    lambda M: M.spmv(_buf(10, np.float32), _buf(7, np.float32), fl.KERNEL_PB, fl.VAL_BF16X2),  # y must be f64
+    lambda M: M.spmv(_buf(10, np.float32), _buf(7, np.float64), fl.KERNEL_FAST, fl.VAL_BF16X2,
+                     work=_buf(13, np.float32)),  # work needs 2m floats
+    lambda M: M.spmm(_buf(10 * 8 - 1, np.float32), _buf(7 * 8, np.float32), 8),  # X needs n*k
+    lambda M: M.spmm(_buf(10 * 8, np.float32), _buf(7 * 4, np.float32), 8),      # Y sized for k=4
+    lambda M: M.spmm(_buf(10, np.float64), _buf(7, np.float64), 1, fl.VAL_BF16),  # kind
+    lambda M: M.permute_x(_buf(10, np.float64), _buf(10, np.float32)),
+    lambda M: M.permute_x(_buf(9, np.float32), _buf(10, np.float32)),
+])
+def test_wrappers_refuse_bad_buffers(call):
+    """FastLoadMatrix.spmv/spmm/permute_x compare buffer sizes and element types with the matrix
+    and the value kind before any pointer reaches the library."""
+    with pytest.raises(fl.FastLoadError) as e:
+        call(_fake_matrix(7, 10))
+    assert e.value.rc == -8

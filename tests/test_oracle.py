@@ -307,3 +307,53 @@ def test_bf16_split_matches_torch():
     nz = v != 0
     rel = np.abs((hif.astype(np.float64) + lof) - v)[nz] / np.abs(v[nz])
     assert rel.max() <= 2.0 ** -16
+
+
+# --------------------------------------------------------- pins of the remaining oracle functions
+@pytest.mark.parametrize("a", [[], [5], [3, 0, 4, 1, 1, 0, 9], list(range(100, 0, -3))])
+def test_exclusive_scan_matches_cumsum(a):
+    """or_exclusive_scan (AUX:29-44) against numpy: out[0] = 0, out[i] = sum(a[:i])."""
+    a = np.asarray(a, np.int32)
+    want = np.concatenate([[0], np.cumsum(a)[:-1]]).astype(np.int32) if a.size else a
+    assert np.array_equal(ob.exclusive_scan(a), want)
+    rng = np.random.default_rng(7)
+    b = rng.integers(0, 1000, 5000).astype(np.int32)
+    assert np.array_equal(ob.exclusive_scan(b), np.concatenate([[0], np.cumsum(b)[:-1]]))
+
+
+def _tiles_of(g, W):
+    return dict(tilenum=int(g[f"w{W}_tilenum"]), tile_ptr=g[f"w{W}_tile_ptr"], tile_len=g[f"w{W}_tile_len"],
+                tile_colidx=g[f"w{W}_tile_colidx"], sortrowidx=g[f"w{W}_sortrowidx"], sortval=g[f"w{W}_sortval"])
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_fastload_cpu_f32_integer_data_matches_reference_y(W):
+    """or_fastload_cpu_f32 on the FP32-exact integer data of src/main.cu:68-69,93-96 (values
+    i % 10, x i % 10, every partial sum an integer below 2^24) equals the y the reference's own
+    FastLoad_cpu produced in FP64 (the committed fixture), bit for bit."""
+    g = np.load(GOLDEN / "ref_layout_1138_bus.npz")
+    m = int(g["m"])
+    t = _tiles_of(g, W)
+    y = ob.fastload_cpu_f32(t, t["sortval"].astype(np.float32), g["sortx"].astype(np.float32), m)
+    assert np.abs(g[f"w{W}_y"]).max() < 2 ** 24
+    assert np.array_equal(y.astype(np.float64), g[f"w{W}_y"])
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_fastload_cpu_f32_matches_serial_float32_loop(W):
+    """or_fastload_cpu_f32 on random data against a float32 numpy loop in tile order: every
+    product and every sum rounded to FP32, in the order of FC:19-31 -- bit for bit."""
+    g = np.load(GOLDEN / "ref_layout_symgeom.npz")
+    m = int(g["m"])
+    t = _tiles_of(g, W)
+    rng = np.random.default_rng(W)
+    v = rng.standard_normal(t["sortval"].size).astype(np.float32)
+    x = rng.standard_normal(g["sortx"].size).astype(np.float32)
+    want = np.zeros(m, np.float32)
+    for i in range(t["tilenum"]):
+        s0, s1, ln, cx = int(t["tile_ptr"][i]), int(t["tile_ptr"][i + 1]), int(t["tile_len"][i]), int(t["tile_colidx"][i])
+        for j in range(s0, s1):
+            r = t["sortrowidx"][j]
+            want[r] = np.float32(want[r] + np.float32(v[j] * x[cx + (j - s0) % ln]))
+    got = ob.fastload_cpu_f32(t, v, x, m)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
