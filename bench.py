@@ -99,8 +99,11 @@ def profiled_traffic(schedule: str, kind: str, workload: str):
     return None
 
 
-def time_variant(fl, run, dy, steps, warmup, zero=True):
-    """Step = (y = 0) + run(); events bracket every run() (the SpMV launches) and the whole loop."""
+def time_variant(fl, run, dy, steps, warmup, zero=True, flush=None):
+    """Step = (y = 0) + run(); events bracket every run() (the SpMV launches) and the whole loop.
+    With `flush` (a device buffer larger than the 256 MB Infinity Cache), a fill kernel over it
+    runs before every rep, outside the run() events, so no rep finds the previous rep's data in
+    the cache; the loop time then includes the fills and only the per-run times are meaningful."""
     from fastload_amd import Event
     for _ in range(warmup):
         if zero:
@@ -111,6 +114,8 @@ def time_variant(fl, run, dy, steps, warmup, zero=True):
     t0, t1 = Event(), Event()
     t0.record()
     for a, b in ev:
+        if flush is not None:
+            fl.zero_device(flush)
         if zero:
             fl.zero_device(dy)
         a.record()
@@ -148,8 +153,9 @@ def select_schedule(fl, m, n, dcp, dri, dv, top=None, force=None):
                                   tuning_s=round(ms / 1e3, 3), trained=s.mlp is not None)
 
 
-def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
-    """Times `variants` ({name: (W, kernel, kind, s_val, bt, R)}) on the resident CSC matrix."""
+def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup, flush=None):
+    """Times `variants` ({name: (W, kernel, kind, s_val, bt, R)}) on the resident CSC matrix; with
+    `flush`, every variant is timed a second time with the Infinity Cache flushed before each rep."""
     from fastload_amd import DeviceArray
     nnz = dri.count
     mats = {}
@@ -200,8 +206,11 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
                 M.spmv(dsx, ydev, kernel, kind, assign=assign)
         fl.synchronize()
         prep_s = time.time() - t0
-        step_ms, kern_ms, kern_med = time_variant(fl, run, ydev, steps, warmup,
-                                                  zero=kernel >= 0 and not assign)
+        zero = kernel >= 0 and not assign
+        step_ms, kern_ms, kern_med = time_variant(fl, run, ydev, steps, warmup, zero=zero)
+        flushed = None
+        if flush is not None:
+            _, flushed, _ = time_variant(fl, run, ydev, steps, 1, zero=zero, flush=flush)
         y = ydev.to_host().astype(np.float64)
         err = float(np.linalg.norm(y - y_ref) / ref_norm) if ref_norm else 0.0
         s_y = 8 if ydev is y64 else 4
@@ -212,23 +221,73 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup):
                           gbs_equiv_f32=round(B32 / step_ms / 1e6, 2), kernel_gbs_equiv_f32=round(B32 / kern_ms / 1e6, 2),
                           bytes_alg=B, rel_err_l2=err, gflops=round(2 * nnz / step_ms / 1e6, 2),
                           schedule_prep_s=round(prep_s, 3))
+        if flushed is not None:
+            res[vname].update(kernel_ms_flushed=round(flushed, 5), kernel_gbs_flushed=round(B / flushed / 1e6, 2),
+                              kernel_gbs_equiv_f32_flushed=round(B32 / flushed / 1e6, 2))
         log(f"[{name}] {vname:14s} step {step_ms:.4f} ms kernel {kern_ms:.4f} ms  {res[vname]['gbs']:7.1f} GB/s "
-            f"(f32-equiv {res[vname]['gbs_equiv_f32']:7.1f})  err {err:.2e}")
-    return nnz, mats, res
+            f"(f32-equiv {res[vname]['gbs_equiv_f32']:7.1f})  err {err:.2e}"
+            + (f"  flushed kernel {flushed:.4f} ms" if flushed is not None else ""))
+    return nnz, mats, res, y_ref
+
+
+def run_comparators(fl, name, m, n, dcp, dri, dv, x, y_ref, steps, warmup, flush=None):
+    """Measurement-only yardsticks on the same resident matrix: the reference's naive CSC kernel
+    (src/CSCSpMV.h:4-27: a lane per column, an atomic per entry) and rocSPARSE's adaptive CSR SpMV
+    (the vendor library, in place of the reference's cuSPARSE call, src/cusparse_cuda.h), FP32 and
+    FP64. Neither is a FastLoad schedule; they are timed like the variants (events around each
+    launch; y cleared by a fill kernel first for the accumulating naive kernel)."""
+    from fastload_amd import DeviceArray
+    from fastload_amd import compare as cmpl
+    nnz = dri.count
+    cp, ri, v = dcp.to_host(), dri.to_host(), dv.to_host()
+    # CSR of A = CSC of A^T: the library's own transposition, treating A's columns as rows
+    cv, cr, ccp, _, _ = fl.csr_to_csc(n, m, v, cp, ri)
+    del cp, ri, v
+    ref_norm = float(np.linalg.norm(y_ref))
+    res = {}
+    for dt, tag in ((np.float32, "f32"), (np.float64, "f64")):
+        s = np.dtype(dt).itemsize
+        dval = dv if dt == np.float64 else DeviceArray.from_host(dv.to_host().astype(np.float32))
+        dx = DeviceArray.from_host(x.astype(dt))
+        dy = DeviceArray(m, dt)
+        B = b_alg(nnz, n, m, s, s, s)
+        R = cmpl.RocsparseCSR(m, n, DeviceArray.from_host(ccp), DeviceArray.from_host(cr),
+                              DeviceArray.from_host(cv.astype(dt)))
+        for cname, run, zero in ((f"naive_csc_{tag}", lambda: cmpl.csc_naive_device(m, n, dcp, dri, dval, dx, dy), True),
+                                 (f"rocsparse_csr_{tag}", lambda: R.run(dx, dy), False)):
+            step_ms, kern_ms, kern_med = time_variant(fl, run, dy, steps, warmup, zero=zero)
+            flushed = None
+            if flush is not None:
+                _, flushed, _ = time_variant(fl, run, dy, steps, 1, zero=zero, flush=flush)
+            err = float(np.linalg.norm(dy.to_host().astype(np.float64) - y_ref) / ref_norm) if ref_norm else 0.0
+            res[cname] = dict(step_ms=round(step_ms, 5), kernel_ms=round(kern_ms, 5),
+                              kernel_median_ms=round(kern_med, 5), kernel_gbs=round(B / kern_ms / 1e6, 2),
+                              bytes_alg=B, rel_err_l2=err)
+            if flushed is not None:
+                res[cname].update(kernel_ms_flushed=round(flushed, 5), kernel_gbs_flushed=round(B / flushed / 1e6, 2))
+            log(f"[{name}] {cname:18s} kernel {kern_ms:.4f} ms  {res[cname]['kernel_gbs']:7.1f} GB/s  err {err:.2e}"
+                + (f"  flushed {flushed:.4f} ms" if flushed is not None else ""))
+        del R
+    return res
 
 
 def run_batched(fl, count, seed, steps, warmup):
     """Batched configuration: `count` square CSC matrices, n log-uniform in [1e3, 2e5], column nnz
     Zipf(2.0) truncated at n, FP32, k = 1, 8, 32 right-hand sides. The batch runs as one
     block-diagonal matrix (k = 1: the schedule the learned selector picks for it; k > 1: the
-    row-ordered multi-RHS kernel) captured in a hipGraph, against
-    `count` sequential per-matrix launches (plain drop-in FP32 kernel for k = 1, per-matrix ROW
-    kernel for k > 1)."""
+    row-ordered multi-RHS kernel) captured in a hipGraph. Baselines, each `count` sequential
+    per-matrix launches (eager):
+      sequential_plain   the plain FP32 drop-in per matrix -- for k > 1 once per right-hand side
+                         (the reference kernel multiplies one vector per launch): count * k launches
+      sequential_same    the batch's own schedule run on each matrix alone (k = 1: the learned
+                         schedule; k > 1: the row kernel), so the batching gain is separate from
+                         the schedule gain"""
     from fastload_amd import DeviceArray, Event, Graph, Stream
+    from fastload_amd import workloads as wl
     rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), count)).astype(int)
     t0 = time.time()
-    mats = [fl.gen_matrix(fl.GEN_ZIPF, seed * 100000 + i, int(n), int(n), (2.0, int(n))) for i, n in enumerate(sizes)]
+    mats = wl.batch(count, seed)
+    sizes = np.array([m for (m, *_r) in mats])
     M_, N_, cp, ri, v, ro, co = fl.batch_assemble(mats)
     nnz = int(cp[-1])
     log(f"[batched] {count} matrices, rows {sizes.min()}..{sizes.max()}, sum rows {M_}, nnz {nnz} "
@@ -241,14 +300,15 @@ def run_batched(fl, count, seed, steps, warmup):
     B = fl.FastLoadMatrix(M_, N_, cp, ri, v, width=64)
     B.prepare_rows()
     B1 = B if sched.width == 64 else fl.FastLoadMatrix(M_, N_, cp, ri, v, width=sched.width)
+    batched_sched = sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB)
     if sched.kernel >= 0:
         B1.prepare(fl.VAL_F32)
-    if sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
+    if batched_sched:
         B1.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
     res = {"matrices": count, "rows": int(M_), "nnz": nnz, "learned_schedule_k1": chosen}
 
-    def timed(fn, reps):
-        for _ in range(warmup):
+    def timed(fn, reps, warm=None):
+        for _ in range(warmup if warm is None else warm):
             fn()
         s.synchronize()
         a, b = Event(), Event()
@@ -259,14 +319,20 @@ def run_batched(fl, count, seed, steps, warmup):
         s.synchronize()
         return a.elapsed_ms(b) / reps
 
-    # per-matrix objects for the sequential baseline
+    # per-matrix objects for the sequential baselines: the 32-wide drop-in layout (plain) and the
+    # learned schedule's layout (same)
     singles = []
     for (m, n, c, r, vv) in mats:
         P = fl.FastLoadMatrix(m, n, c, r, vv, width=32)
         P.prepare(fl.VAL_F32)
         Q = fl.FastLoadMatrix(m, n, c, r, vv, width=64)
         Q.prepare_rows()
-        singles.append((P, Q))
+        S = Q if sched.width == 64 else fl.FastLoadMatrix(m, n, c, r, vv, width=sched.width)
+        if sched.kernel >= 0:
+            S.prepare(fl.VAL_F32)
+        if batched_sched:
+            S.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
+        singles.append((P, Q, S, P.perm(), S.perm() if sched.kernel >= 0 else None))
     log(f"[batched] per-matrix preprocessing done ({time.time() - t0:.1f}s)")
     for k in (1, 8, 32):
         X = rng.standard_normal((N_, k)).astype(np.float32)
@@ -284,59 +350,113 @@ def run_batched(fl, count, seed, steps, warmup):
                 B.spmm(dX, dY, k, fl.VAL_F32, stream=s)
         batch_step()
         s.synchronize()
-        y_batch = dY.to_host()
+        y_batch = dY.to_host().reshape(M_, k)
         g = Graph(s)
         with g:
             batch_step()
         t_graph = timed(g.launch, steps)
         t_eager = timed(batch_step, steps)
-        # sequential per-matrix launches
-        xs, ys, sxs = [], [], []
-        for i, (P, Q) in enumerate(singles):
-            xi = X[co[i]:co[i + 1]]
-            xs.append(DeviceArray.from_host(np.ascontiguousarray(xi).ravel()))
-            ys.append(DeviceArray(mats[i][0] * k, np.float32))
-            if k == 1:
-                sxs.append(DeviceArray(mats[i][1], np.float32))
-                P.permute_x(xs[-1], sxs[-1])
+        # plain drop-in, one launch per matrix and right-hand side: x columns permuted on the host,
+        # matrix i's column j at offset co[i] * k + j * n_i (likewise y)
+        XP = np.empty(N_ * k, np.float32)
+        for i, (P, Q, S, perm, _) in enumerate(singles):
+            XP[co[i] * k:co[i + 1] * k] = X[co[i]:co[i + 1]][perm].T.ravel()
+        dXP = DeviceArray.from_host(XP)
+        dYP = DeviceArray(M_ * k, np.float32)
+        plain = []
+        for i, (P, Q, S, perm, _) in enumerate(singles):
+            mi, ni = mats[i][0], mats[i][1]
+            for j in range(k):
+                plain.append((P, dXP.view(co[i] * k + j * ni, ni), dYP.view(ro[i] * k + j * mi, mi)))
 
-        def seq_step():
-            for i, (P, Q) in enumerate(singles):
-                if k == 1:
-                    fl.zero_device(ys[i], s)
-                    P.spmv(sxs[i], ys[i], fl.KERNEL_DROPIN, fl.VAL_F32, stream=s)
+        def seq_plain():
+            for P, sx, y in plain:
+                fl.zero_device(y, s)
+                P.spmv(sx, y, fl.KERNEL_DROPIN, fl.VAL_F32, stream=s)
+        t_plain = timed(seq_plain, max(steps // 10, 3), warm=min(warmup, 2))
+        YP = dYP.to_host()
+        y_plain = np.concatenate([YP[ro[i] * k:ro[i + 1] * k].reshape(k, -1).T for i in range(count)])
+        # the batch's own schedule per matrix
+        same = []
+        for i, (P, Q, S, perm, sperm) in enumerate(singles):
+            mi, ni = mats[i][0], mats[i][1]
+            xi = dX.view(co[i] * k, ni * k)
+            yi = dY.view(ro[i] * k, mi * k)
+            if k == 1 and sched.kernel >= 0:
+                sxi = DeviceArray.from_host(X[co[i]:co[i + 1], 0][sperm])
+                same.append((S, sxi, yi, True))
+            else:
+                same.append((Q, xi, yi, False))
+
+        def seq_same():
+            for S, x_, y_, spmv in same:
+                if spmv:
+                    S.spmv(x_, y_, sched.kernel, fl.VAL_F32, stream=s, assign=True)
                 else:
-                    Q.spmm(xs[i], ys[i], k, fl.VAL_F32, stream=s)
-        t_seq = timed(seq_step, max(steps // 10, 3))
-        y_seq = np.concatenate([y.to_host() for y in ys])
-        err = float(np.linalg.norm(y_batch.astype(np.float64) - y_seq) / max(np.linalg.norm(y_seq), 1e-300))
-        res[f"k{k}"] = dict(graph_ms=round(t_graph, 4), eager_ms=round(t_eager, 4), sequential_ms=round(t_seq, 4),
-                            speedup_vs_sequential=round(t_seq / t_graph, 2), graph_gbs=round(B_alg / t_graph / 1e6, 1),
-                            sequential_gbs=round(B_alg / t_seq / 1e6, 1), bytes_alg=B_alg,
-                            rel_diff_batch_vs_sequential=err,
-                            schedule=(f"{sched.name} (learned) " if k == 1 else "row ") + "on the block-diagonal batch",
-                            sequential="plain FP32 drop-in per matrix" if k == 1 else "ROW per matrix")
-        log(f"[batched] k={k:2d} graph {t_graph:.3f} ms  eager {t_eager:.3f} ms  sequential {t_seq:.3f} ms  "
-            f"({res[f'k{k}']['graph_gbs']} GB/s, x{res[f'k{k}']['speedup_vs_sequential']})  diff {err:.1e}")
-        del g
+                    S.spmm(x_, y_, k, fl.VAL_F32, stream=s)
+        t_same = timed(seq_same, max(steps // 10, 3), warm=min(warmup, 2))
+        y_same = dY.to_host().reshape(M_, k)
+        nb = max(np.linalg.norm(y_plain), 1e-300)
+        err_plain = float(np.linalg.norm(y_batch.astype(np.float64) - y_plain) / nb)
+        same_bitwise = bool(np.array_equal(y_batch.view(np.uint32), y_same.view(np.uint32)))
+        err_same = float(np.linalg.norm(y_batch.astype(np.float64) - y_same) / nb)
+        sched_name = (f"{sched.name} (learned)" if k == 1 else "row")
+        res[f"k{k}"] = dict(graph_ms=round(t_graph, 4), eager_ms=round(t_eager, 4), graph_gbs=round(B_alg / t_graph / 1e6, 1),
+                            bytes_alg=B_alg, schedule=f"{sched_name} on the block-diagonal batch",
+                            sequential_plain_ms=round(t_plain, 4),
+                            sequential_plain=f"plain FP32 drop-in, {count * k} launches ({count} matrices x {k} "
+                                             f"right-hand sides), eager",
+                            speedup_vs_sequential_plain=round(t_plain / t_graph, 2),
+                            rel_diff_vs_sequential_plain=err_plain,
+                            sequential_same_ms=round(t_same, 4),
+                            sequential_same=f"{sched_name} per matrix, {count} launches, eager",
+                            speedup_vs_sequential_same=round(t_same / t_graph, 2),
+                            same_bit_identical=same_bitwise, rel_diff_vs_sequential_same=err_same)
+        log(f"[batched] k={k:2d} graph {t_graph:.3f} ms  eager {t_eager:.3f} ms  ({res[f'k{k}']['graph_gbs']} GB/s)  "
+            f"plain x{k} seq {t_plain:.3f} ms (x{res[f'k{k}']['speedup_vs_sequential_plain']}, diff {err_plain:.1e})  "
+            f"same-schedule seq {t_same:.3f} ms (x{res[f'k{k}']['speedup_vs_sequential_same']}, "
+            f"{'bit-identical' if same_bitwise else f'diff {err_same:.1e}'})")
+        del g, plain, same
     return res
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_threads() -> int:
+    """Cores this process may use, capped at the 16 a one-GPU host grants a job."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
 def cpu_baseline(M, m, n, nnz, budget_s=10.0):
-    """The reference's own CPU path, FastLoad_cpu (src/FastLoad_CPU.h:2-46), compiled from the
-    reference sources into oracle/_ref, over the same 32-wide layout in FP64 (its only
-    precision), repeated for about budget_s seconds on one core; y is zeroed before each pass,
-    as a GPU step does. Falls back to the oracle's serial FP32 port of the same loop when
-    oracle/_ref was not built. `value` is in the headline's unit, GB/s-equivalent with FP32
-    bytes (nnz*8 + n*4 + m*4 per pass)."""
+    """The CPU path over the same 32-wide layout, y zeroed before each pass as a GPU step does:
+      - the reference's own FastLoad_cpu (src/FastLoad_CPU.h:2-46), compiled from the reference
+        sources into oracle/_ref, FP64 (its only precision), one core, about budget_s seconds
+        (falls back to the oracle's serial FP32 port when oracle/_ref was not built);
+      - this library's fl_FastLoad_cpu on every core the process may use (tiles split into
+        ranges, atomic y), FP64, about budget_s seconds.
+    `value` is in the headline's unit, GB/s-equivalent with FP32 bytes (nnz*8 + n*4 + m*4)."""
     import ctypes as C
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
     import oracle_bind as ob
     e = M.export()
     x = np.random.default_rng(0).standard_normal(n)
     B32 = b_alg(nnz, n, m, 4, 4, 4)
+    sx = np.ascontiguousarray(x[e["perm"]], np.float64)
     L = ob.ref_lib(32) if ob.ref_available(32) else None
     reps, t_total = 0, 0.0
+    out = None
     if L is not None:
         keep = dict(sortval=np.ascontiguousarray(e["sortval"], np.float64),
                     sortrowidx=np.ascontiguousarray(e["sortrowidx"], np.int32),
@@ -348,7 +468,6 @@ def cpu_baseline(M, m, n, nnz, budget_s=10.0):
         for k in ("sortrowidx", "tile_ptr", "tile_len", "tile_colidx"):
             setattr(sm, k, keep[k].ctypes.data_as(ob._IP))
         sm.tilenum = int(e["tilenum"])
-        sx = np.ascontiguousarray(x[e["perm"]], np.float64)
         y = np.zeros(m)
         while t_total < budget_s and reps < 100:
             t0 = time.perf_counter()
@@ -357,25 +476,102 @@ def cpu_baseline(M, m, n, nnz, budget_s=10.0):
             t_total += time.perf_counter() - t0
             reps += 1
         per = t_total / reps
-        return dict(value=round(B32 / per / 1e9, 3), unit="GB/s", cores=1, kind="reference",
-                    sample=f"{reps} full FastLoad_cpu passes (reference source, FP64, y zeroed each pass) over the "
-                           f"same 32-wide layout, {per * 1e3:.1f} ms each; GB/s-equivalent with FP32 bytes",
-                    fp64_bytes_gbs=round(b_alg(nnz, n, m, 8, 8, 8) / per / 1e9, 3),
-                    cpu=platform.processor() or platform.machine())
-    t = dict(tilenum=e["tilenum"], tile_ptr=e["tile_ptr"], tile_len=e["tile_len"], tile_colidx=e["tile_colidx"],
-             sortrowidx=e["sortrowidx"])
-    v32 = e["sortval"].astype(np.float32)
-    x32 = x.astype(np.float32)
+        y_ref1 = y.copy()
+        out = dict(value=round(B32 / per / 1e9, 3), unit="GB/s", cores=1, kind="reference",
+                   sample=f"{reps} full FastLoad_cpu passes (reference source, FP64, y zeroed each pass) over the "
+                          f"same 32-wide layout, {per * 1e3:.1f} ms each; GB/s-equivalent with FP32 bytes",
+                   ms_per_spmv=round(per * 1e3, 3), fp64_bytes_gbs=round(b_alg(nnz, n, m, 8, 8, 8) / per / 1e9, 3))
+    else:
+        t = dict(tilenum=e["tilenum"], tile_ptr=e["tile_ptr"], tile_len=e["tile_len"], tile_colidx=e["tile_colidx"],
+                 sortrowidx=e["sortrowidx"])
+        v32 = e["sortval"].astype(np.float32)
+        x32 = sx.astype(np.float32)
+        while t_total < budget_s and reps < 100:
+            t0 = time.perf_counter()
+            ob.fastload_cpu_f32(t, v32, x32, m)
+            t_total += time.perf_counter() - t0
+            reps += 1
+        per = t_total / reps
+        y_ref1 = None
+        out = dict(value=round(B32 / per / 1e9, 3), unit="GB/s", cores=1, kind="port",
+                   sample=f"{reps} full FastLoad_cpu FP32 passes (oracle port) over the same 32-wide layout "
+                          f"({per * 1e3:.1f} ms each)", ms_per_spmv=round(per * 1e3, 3))
+    out["cpu"] = cpu_model()
+    # this library's host path on every available core
+    import fastload_amd as fl
+    tiles = fl.Tiles(int(e["tilenum"]), int(e["segsum"]), e["tile_ptr"], e["tile_len"], e["tile_colidx"],
+                     e["tile_format"], e["sortrowidx"], e["sortval"], e["sortrowindex"], e["countrow"],
+                     e["segmentoffset"])
+    T = cpu_threads()
+    y = np.zeros(m)
+    reps, t_total = 0, 0.0
     while t_total < budget_s and reps < 100:
         t0 = time.perf_counter()
-        ob.fastload_cpu_f32(t, v32, x32, m)
+        y[:] = 0.0
+        fl.FastLoad_cpu(tiles, nnz, m, n, sx, threads=T, y=y)
         t_total += time.perf_counter() - t0
         reps += 1
     per = t_total / reps
-    return dict(value=round(B32 / per / 1e9, 3), unit="GB/s", cores=1, kind="port",
-                sample=f"{reps} full FastLoad_cpu FP32 passes (oracle port) over the same 32-wide layout "
-                       f"({per * 1e3:.1f} ms each)",
-                cpu=platform.processor() or platform.machine())
+    diff = float(np.linalg.norm(y - y_ref1) / max(np.linalg.norm(y_ref1), 1e-300)) if y_ref1 is not None else None
+    out["all_cores"] = dict(value=round(B32 / per / 1e9, 3), unit="GB/s", cores=T, ms_per_spmv=round(per * 1e3, 3),
+                            impl="fl_FastLoad_cpu (this library): tile ranges per thread, atomic y updates, FP64",
+                            sample=f"{reps} full passes, y zeroed each pass", rel_diff_vs_1core=diff)
+    log(f"[cpu] 1 core {out['ms_per_spmv']} ms/SpMV ({out['value']} GB/s); {T} cores {per * 1e3:.1f} ms/SpMV "
+        f"({out['all_cores']['value']} GB/s) on {out['cpu']}")
+    return out
+
+
+PUBLISHED_1138_MS = 0.066730  # src/TimeResult/results_slidecuda.txt:1 (RTX 3090 Ti per readme.md:18)
+
+
+def run_c1(fl, steps, warmup):
+    """Plain FastLoad FP64 on 1138_bus (values i % 10 and x i % 10, as src/main.cu:68-69,93-96 set
+    them) and on the seeded structurally symmetric twin. Each goes through the reference's host API
+    (mmio_allinone / csr_to_csc / ColSort / formatTransform / FastLoad_cpu / FastLoad_spmv): the
+    reference measures one FastLoad_spmv launch as the host clock around launch + synchronize,
+    mean of 100 (src/FastLoad_GPU.h:349-374), and that is the number it publishes for 1138_bus
+    (0.066730 ms). The device-resident drop-in kernel is also timed with HIP events."""
+    from fastload_amd import DeviceArray
+    from fastload_amd import workloads as wl
+    root = os.path.dirname(os.path.abspath(__file__))
+    cases = {}
+    m, n, nnz, _, rp, ci, v = fl.mmio_allinone(os.path.join(root, "tests", "golden", "1138_bus.mtx"))
+    v = (np.arange(nnz) % 10).astype(np.float64)
+    cases["1138_bus"] = (m, n, rp, ci, v, (np.arange(n) % 10).astype(np.float64))
+    m2, n2, cp2, ri2, v2, x2 = wl.symgeom()
+    import scipy.sparse as sp
+    A = sp.csc_matrix((v2, ri2, cp2), shape=(m2, n2)).tocsr()
+    A.sort_indices()
+    cases["symgeom_twin"] = (m2, n2, A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data, x2)
+    out = {}
+    for name, (m, n, rp, ci, v, x) in cases.items():
+        cv, cr, cp, npc, _ = fl.csr_to_csc(m, n, v, rp, ci)
+        tr, tv, sn, sx, _ = fl.ColSort(m, n, npc, cp, cr, cv, x)
+        tiles, h, _, _ = fl.formatTransform(tr, tv, sn, len(cv), m, n)
+        y_gold = np.zeros(m)
+        for c in range(n):  # y_goldencsc, src/main.cu:110-120
+            y_gold[cr[cp[c]:cp[c + 1]]] += cv[cp[c]:cp[c + 1]] * x[c]
+        t0 = time.perf_counter()
+        y_cpu, cpu_err = fl.FastLoad_cpu(tiles, len(cv), m, n, sx, y_golden=y_gold if name == "1138_bus" else None)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        y, wall_ms, passed = fl.FastLoad_spmv(tiles, len(cv), n, m, sx, y_golden=y_gold)
+        M = fl.FastLoadMatrix(m, n, cp, cr, cv, width=32)
+        dsx = DeviceArray.from_host(sx)
+        dy = DeviceArray(m, np.float64)
+        _, kern_ms, kern_med = time_variant(fl, lambda: M.spmv(dsx, dy, fl.KERNEL_DROPIN, fl.VAL_F64), dy,
+                                            steps, warmup)
+        rel = float(np.linalg.norm(dy.to_host() - y_gold) / max(np.linalg.norm(y_gold), 1e-300))
+        out[name] = dict(m=m, n=n, nnz=int(len(cv)), levels=h, tiles=tiles.tilenum, fp64=True,
+                         fastload_spmv_ms_reference_method=round(wall_ms, 5), check="PASS" if passed else "NO PASS",
+                         kernel_event_ms_mean=round(kern_ms, 5), kernel_event_ms_median=round(kern_med, 5),
+                         rel_err_vs_golden=rel, fastload_cpu_ms=round(cpu_ms, 4),
+                         fastload_cpu_exact=(cpu_err == 0) if name == "1138_bus" else None)
+        if name == "1138_bus":
+            out[name].update(published_ms=PUBLISHED_1138_MS, published_hw="RTX 3090 Ti (readme.md:18), wall clock",
+                             speedup_vs_published=round(PUBLISHED_1138_MS / wall_ms, 2))
+        log(f"[c1] {name}: FastLoad_spmv {wall_ms:.4f} ms (reference method), kernel {kern_ms * 1e3:.1f} us "
+            f"(events), FastLoad_cpu {cpu_ms:.3f} ms, {out[name]['check']}")
+    return out
 
 
 def main():
@@ -387,6 +583,9 @@ def main():
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timings")
+    ap.add_argument("--no-compare", action="store_true", help="skip the naive-CSC and rocSPARSE yardsticks")
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
     ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
     ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,winr64_f32,pb_f32,row_f32")
@@ -397,16 +596,16 @@ def main():
                     help="value storages timed for the learned schedule")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this FastLoad build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "this FastLoad build runs on one GPU only (no multi-GPU path is built); "
+                                   "run with --gpus 1"}))
         sys.exit(2)
 
     import fastload_amd as fl
     from fastload_amd import DeviceArray
+    from fastload_amd import workloads as wl
 
-    n = m = args.n
     t0 = time.time()
-    _, _, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, args.seed, m, n, (1.8423, 50000))
-    x = fl.gen_vector(fl.VEC_NORMAL, args.seed + 1, n)
+    m, n, cp, ri, v, x = wl.powerlaw(args.n, args.seed)
     log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
     dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
     del cp, ri, v
@@ -414,7 +613,12 @@ def main():
     log(f"[powerlaw] selector chose {chosen}")
     variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
     variants.update(learned_variants(sched, args.learned_kinds.split(",")))
-    nnz, mats, res = run_workload(fl, "powerlaw", m, n, dcp, dri, dv, x, variants, args.steps, args.warmup)
+    flush = None if args.no_flush else DeviceArray(1 << 28, np.float32)  # 1 GiB, 4x the Infinity Cache
+    nnz, mats, res, y_ref = run_workload(fl, "powerlaw", m, n, dcp, dri, dv, x, variants, args.steps, args.warmup,
+                                         flush)
+    del mats
+    comparators = {} if args.no_compare else {
+        "powerlaw": run_comparators(fl, "powerlaw", m, n, dcp, dri, dv, x, y_ref, args.steps, args.warmup, flush)}
     del dcp, dri, dv
 
     ok = {k: r for k, r in res.items() if r["rel_err_l2"] <= 1e-4 and k.startswith("learned")}
@@ -436,20 +640,25 @@ def main():
         "vs_baseline": None,
         "dtype": kind_names[best.split("_", 1)[1]],
         "data": "synthetic",
-        "config": {"workload": f"power-law CSC SpMV {m}x{n}, Zipf(1.8423, max 50000) column nnz, seed {args.seed}",
+        "config": {"workload": f"power-law CSC SpMV {m}x{n}, Zipf({wl.POWERLAW_ZIPF}, max {wl.POWERLAW_MAX}) column "
+                               f"nnz, seed {args.seed}",
                    "nnz": nnz, "variant": best, "learned_schedule": chosen, "rel_err_l2_vs_fp64": r["rel_err_l2"],
                    "plain_f32_gbs": plain["gbs"] if plain else None,
                    "speedup_vs_plain_f32": round(plain["step_ms"] / r["step_ms"], 3) if plain else None},
         "roofline": {"bound": "hbm", "achieved": r["kernel_gbs_equiv_f32"], "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4),
-                     "traffic": profiled_traffic(chosen["schedule"], best.split("_", 1)[1], "powerlaw")},
+                     "traffic": profiled_traffic(chosen["schedule"], best.split("_", 1)[1], "powerlaw"),
+                     "flushed_achieved": r.get("kernel_gbs_equiv_f32_flushed"),
+                     "flushed_frac": (round(r["kernel_gbs_equiv_f32_flushed"] / PEAK_GBS, 4)
+                                      if "kernel_gbs_equiv_f32_flushed" in r else None)},
         "variants": res,
     }
-    del mats
+    if comparators:
+        lf = r["kernel_ms"]
+        for k, c in comparators["powerlaw"].items():
+            c["learned_speedup"] = round(c["kernel_ms"] / lf, 2)
     if not args.no_stencil:
-        _, _, scp, sri, sv = fl.gen_matrix(fl.GEN_STENCIL27, 128, 0, 0, (128,))
-        sn = sm = 128 ** 3
-        sx = fl.gen_vector(fl.VEC_NORMAL, 129, sn)
+        sm, sn, scp, sri, sv, sx = wl.stencil(128)
         dcp, dri, dv = DeviceArray.from_host(scp), DeviceArray.from_host(sri), DeviceArray.from_host(sv)
         del scp, sri, sv
         ssched, schosen = select_schedule(fl, sm, sn, dcp, dri, dv, force=args.stencil_schedule)
@@ -457,16 +666,29 @@ def main():
         svars = {k: VARIANTS[k] for k in args.stencil_variants.split(",") if k}
         svars.update(learned_variants(ssched, args.learned_kinds.split(",")))
         svars.update({k: VARIANTS[k] for k in ("win_bf16x2", "win_bf16")})
-        snnz, smats, sres = run_workload(fl, "stencil", sm, sn, dcp, dri, dv, sx, svars, args.steps, args.warmup)
+        snnz, smats, sres, sy_ref = run_workload(fl, "stencil", sm, sn, dcp, dri, dv, sx, svars, args.steps,
+                                                 args.warmup, flush)
+        del smats
+        if not args.no_compare:
+            comparators["stencil"] = run_comparators(fl, "stencil", sm, sn, dcp, dri, dv, sx, sy_ref, args.steps,
+                                                     args.warmup, flush)
+            slf = sres["learned_f32"]["kernel_ms"]
+            for c in comparators["stencil"].values():
+                c["learned_speedup"] = round(c["kernel_ms"] / slf, 2)
         out["stencil"] = {"workload": "27-point stencil on 128^3, |v| U[0.5,1.5] random sign, seed 128", "nnz": snnz,
                           "learned_schedule": schosen,
                           "traffic_bytes_learned_f32": profiled_traffic(schosen["schedule"], "f32", "stencil"),
                           "variants": sres}
-        del smats, dcp, dri, dv
+        del dcp, dri, dv
+    del flush
+    if comparators:
+        out["comparators"] = comparators
+    if not args.no_c1:
+        out["c1"] = run_c1(fl, args.steps, args.warmup)
     if args.batch > 0:
         out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup)
     if not args.no_cpu:
-        _, _, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, args.seed, m, n, (1.8423, 50000))
+        m, n, cp, ri, v, _ = wl.powerlaw(args.n, args.seed)
         M32 = fl.FastLoadMatrix(m, n, cp, ri, v, width=32)
         del cp, ri, v
         out["cpu_baseline"] = cpu_baseline(M32, m, n, nnz)

@@ -37,6 +37,10 @@ SIGNATURES = {
     "fl_formatTransform": (C.c_int, [f64p, f64p, C.POINTER(SlideMatrix), vp, vp, vp, C.c_int, C.c_int, C.c_int, i32p]),
     "fl_FastLoad_spmv": (C.c_int, [C.c_char_p, C.POINTER(SlideMatrix), C.c_int, C.c_int, C.c_int, vp, vp, vp, f64p]),
     "fl_Tile_destroy": (None, [C.POINTER(SlideMatrix)]),
+    "fl_slide_save": (C.c_int, [C.c_char_p, C.POINTER(SlideMatrix), C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "fl_slide_load": (C.c_int, [C.c_char_p, C.POINTER(SlideMatrix), i32p, i32p, i32p, i32p, C.POINTER(i32p)]),
+    "fl_matrix_save": (C.c_int, [vp, C.c_char_p, vp]),
+    "fl_matrix_load": (C.c_int, [C.POINTER(vp), C.c_char_p, vp]),
     "fl_FastLoad_cpu": (C.c_int, [C.POINTER(SlideMatrix), C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, i32p]),
     "fl_FastLoad_cpu_f32": (C.c_int, [C.POINTER(SlideMatrix), vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int]),
     "fl_device_count": (C.c_int, [i32p]),
@@ -73,6 +77,9 @@ SIGNATURES = {
     "fl_graph_launch": (C.c_int, [vp, vp]),
     "fl_graph_destroy": (C.c_int, [vp]),
     "fl_zero_device": (C.c_int, [vp, C.c_size_t, vp]),
+    "fl_cg_create": (C.c_int, [C.POINTER(vp), vp, C.c_int, C.c_int]),
+    "fl_cg_solve": (C.c_int, [vp, vp, vp, C.c_int, C.c_double, C.c_int, i32p, C.POINTER(C.c_double)]),
+    "fl_cg_destroy": (C.c_int, [vp]),
     "fl_features": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
     "fl_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
     "fl_mlp_destroy": (C.c_int, [vp]),

@@ -179,6 +179,42 @@ def FastLoad_cpu_f32(tiles: Tiles, sortval32, nnz: int, rowA: int, colA: int, x3
     return y[:rowA]
 
 
+def slide_save(path: str, tiles: Tiles, rowA: int, colA: int, nnz: int, h_count: int, perm=None):
+    """Writes a slide_matrix (src/format.h:6-20) with its sizes and column order to `path`
+    (fl_slide_save: the versioned FastLoad matrix file)."""
+    sm, keep = _slide(tiles)
+    keep["countrow"] = np.ascontiguousarray(tiles.countrow, np.int32)
+    keep["segmentoffset"] = np.ascontiguousarray(tiles.segmentoffset, np.int32)
+    sm.countrow = C.cast(keep["countrow"].ctypes.data, i32p)
+    sm.segmentoffset = C.cast(keep["segmentoffset"].ctypes.data, i32p)
+    pm = None if perm is None else np.ascontiguousarray(perm, np.int32)
+    check(lib.fl_slide_save(str(path).encode(), C.byref(sm), rowA, colA, nnz, h_count, _p(pm)), "fl_slide_save")
+    del keep
+
+
+def slide_load(path: str):
+    """Reads the slide_matrix sections of a FastLoad matrix file -> (Tiles, rowA, colA, nnz,
+    h_count, perm or None)."""
+    sm = SlideMatrix()
+    m, n, nnz, h = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    pm = i32p()
+    check(lib.fl_slide_load(str(path).encode(), C.byref(sm), C.byref(m), C.byref(n), C.byref(nnz), C.byref(h),
+                            C.byref(pm)), "fl_slide_load")
+    try:
+        T, S, nz = sm.tilenum, sm.segsum, nnz.value
+        tiles = Tiles(T, S, _copy_out(sm.tile_ptr, T + 1, np.int32), _copy_out(sm.tile_len, T, np.int32),
+                      _copy_out(sm.tile_colidx, T, np.int32), _copy_out(sm.tile_format, T, np.int32),
+                      _copy_out(sm.sortrowidx, nz, np.int32), _copy_out(sm.sortval, nz, np.float64),
+                      _copy_out(sm.sortrowindex, nz, np.int32), _copy_out(sm.countrow, T + 1, np.int32),
+                      _copy_out(sm.segmentoffset, S + 1, np.int32))
+        perm = _copy_out(pm, n.value, np.int32) if pm else None
+    finally:
+        lib.fl_Tile_destroy(C.byref(sm))
+        if pm:
+            lib.fl_free(C.cast(pm, C.c_void_p))
+    return tiles, m.value, n.value, nnz.value, h.value, perm
+
+
 # ---------------------------------------------------------------- device-resident matrix
 def _need(d: DeviceArray, count: int, dtype, what: str):
     """Refuses a device buffer too small or of the wrong element type before any kernel sees it."""
@@ -209,6 +245,26 @@ class FastLoadMatrix:
         self.m, self.n, self.nnz, self.width = m, n, nnz, width
         self.tilenum, self.levels = self.info["tilenum"], self.info["levels"]
 
+    @classmethod
+    def load(cls, path: str, stream=None) -> "FastLoadMatrix":
+        """A matrix read back from a file written by save() (fl_matrix_load)."""
+        h = C.c_void_p()
+        check(lib.fl_matrix_load(C.byref(h), str(path).encode(), stream.ptr if stream is not None else None),
+              "fl_matrix_load")
+        M = object.__new__(cls)
+        M.handle = h.value
+        info = MatrixInfo()
+        check(lib.fl_matrix_get_info(M.handle, C.byref(info)), "fl_matrix_get_info")
+        M.info = {k: getattr(info, k) for k, _ in MatrixInfo._fields_}
+        M.m, M.n, M.nnz, M.width = info.m, info.n, info.nnz, info.width
+        M.tilenum, M.levels = info.tilenum, info.levels
+        return M
+
+    def save(self, path: str, stream=None):
+        """Writes the layout and every prepared schedule to `path` (fl_matrix_save)."""
+        check(lib.fl_matrix_save(self.handle, str(path).encode(), stream.ptr if stream is not None else None),
+              "fl_matrix_save")
+
     def export(self) -> dict:
         i = self.info
         n, T, nz, L, S = i["n"], i["tilenum"], i["nnz"], i["levels"], i["segsum"]
@@ -228,6 +284,13 @@ class FastLoadMatrix:
             out[k] = out[k][:s]
         out.update(tilenum=T, segsum=S, L=L)
         return out
+
+    def perm(self) -> np.ndarray:
+        """The column order (ColSort): sortx = x[perm]."""
+        out = np.zeros(max(self.n, 1), np.int32)
+        nulls = [None] * 13
+        check(lib.fl_matrix_export(self.handle, out.ctypes.data, *nulls), "fl_matrix_export")
+        return out[:self.n]
 
     def prepare(self, val_kind: int, stream=None):
         check(lib.fl_matrix_prepare_values(self.handle, val_kind, stream.ptr if stream is not None else None),
@@ -276,6 +339,43 @@ class FastLoadMatrix:
         if getattr(self, "handle", None):
             lib.fl_matrix_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -------------------------------------------------------------------- iterative solver
+KERNEL_ROW = -1
+
+
+class CG:
+    """Conjugate gradients on a resident symmetric positive definite matrix (fl_cg_*): one
+    prepared schedule (`kernel`, or KERNEL_ROW after prepare_rows) in FP64 or FP32, iterations
+    replayed from a hipGraph in groups of `check_every`."""
+
+    def __init__(self, M: "FastLoadMatrix", kernel: int, val_kind: int = VAL_F64):
+        h = C.c_void_p()
+        check(lib.fl_cg_create(C.byref(h), M.handle, kernel, val_kind), "fl_cg_create")
+        self.h, self.M, self.kind = h.value, M, val_kind
+
+    def solve(self, db: DeviceArray, dx: DeviceArray, max_iter: int = 1000, rtol: float = 1e-10,
+              check_every: int = 10):
+        """x (device, initial guess on entry) -> solution; returns (iterations run, ||r||/||b||)."""
+        dt = np.float64 if self.kind == VAL_F64 else np.float32
+        _need(db, self.M.n, dt, "cg b")
+        _need(dx, self.M.n, dt, "cg x")
+        it, res = C.c_int(), C.c_double()
+        check(lib.fl_cg_solve(self.h, db.ptr, dx.ptr, max_iter, rtol, check_every, C.byref(it), C.byref(res)),
+              "fl_cg_solve")
+        return it.value, res.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.fl_cg_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

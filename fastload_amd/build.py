@@ -1,5 +1,6 @@
-"""Builds fastload_amd/libfastload.so in-tree: HIP sources with hipcc for gfx950, host-only
-sources with g++, linked by hipcc. Run `python fastload_amd/build.py` (or __graft_entry__.build());
+"""Builds fastload_amd/libfastload.so in-tree (csrc/: HIP sources with hipcc for gfx950, host-only
+sources with g++, linked by hipcc) and the measurement-only comparator library
+fastload_amd/libfastload_compare.so (csrc_compare/, linked with rocSPARSE). Run `python fastload_amd/build.py` (or __graft_entry__.build());
 the script does not import the package, so a stale library never blocks a rebuild.
 """
 from __future__ import annotations
@@ -15,6 +16,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libfastload.so"
+CSRC_COMPARE = PKG / "csrc_compare"
+LIB_COMPARE = PKG / "libfastload_compare.so"
 ARCH = os.environ.get("FASTLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -37,7 +40,7 @@ def _run(cmd):
 
 
 def _compile(src: Path) -> Path:
-    obj = OBJ / (src.name + ".o")
+    obj = OBJ / (src.parent.name + "_" + src.name + ".o")
     if src.suffix == ".hip":
         cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
     else:
@@ -46,16 +49,23 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+def _link(out: Path, objs, libs):
+    tmp = out.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), *libs])
+    os.replace(tmp, out)
+
+
 def build(verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
-    os.replace(tmp, LIB)
+    cmp_srcs = sorted(CSRC_COMPARE.glob("*.hip"))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs) + len(cmp_srcs))) as ex:
+        objs = list(ex.map(_compile, srcs + cmp_srcs))
+    _link(LIB, objs[:len(srcs)], ["-lpthread"])
+    # the comparators (measurement only) live in their own library, which libfastload.so never loads
+    _link(LIB_COMPARE, objs[len(srcs):], ["-L/opt/rocm/lib", "-lrocsparse"])
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {LIB_COMPARE}")
     return LIB
 
 

@@ -18,7 +18,6 @@ int check_compressed(const int *d_ptr, int count, const int *d_idx, int nnz, int
 void prepare_values(fl_matrix *A, int kind, hipStream_t s);
 void permute_x(const fl_matrix *A, const void *x, void *sx, bool dbl, hipStream_t s);
 void permute_raw(const int *perm, int n, const void *x, void *sx, bool dbl, hipStream_t s);
-int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s);
 void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s);
 void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s);
 

@@ -100,6 +100,37 @@ void row_prepare(fl_matrix *A, hipStream_t s) {
     A->row_ready = true;
 }
 
+__global__ void k_row_val32(const double *__restrict__ v64, float *__restrict__ v32, int count) {
+    GSTRIDE(i, count) v32[i] = (float)v64[i];
+}
+
+// The row order read back from a file: row_ptr [m+1] must be a pointer array over nnz and every
+// column inside [0, n) before the kernels may index with them.
+int row_install(fl_matrix *A, const int *row_ptr, const int *row_col, const double *row_val, hipStream_t s) {
+    const int m = A->m, n = A->n, nnz = A->nnz;
+    bool ok = row_ptr[0] == 0 && row_ptr[m] == nnz;
+    for (int r = 0; ok && r < m; ++r) ok = row_ptr[r] <= row_ptr[r + 1];
+    for (long long i = 0; ok && i < nnz; ++i) ok = row_col[i] >= 0 && row_col[i] < n;
+    if (!ok) {
+        set_error("fl_matrix_load: row order invalid (row pointers or columns out of range)");
+        return FL_ERR_INVALID;
+    }
+    A->row_ptr.alloc((size_t)m + 1);
+    A->row_col.alloc(std::max(nnz, 1));
+    A->row_val64.alloc(std::max(nnz, 1));
+    A->row_val32.alloc(std::max(nnz, 1));
+    FL_HIP(hipMemcpyAsync(A->row_ptr.p, row_ptr, sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice, s));
+    if (nnz) {
+        FL_HIP(hipMemcpyAsync(A->row_col.p, row_col, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, s));
+        FL_HIP(hipMemcpyAsync(A->row_val64.p, row_val, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice, s));
+        k_row_val32<<<sgb(nnz), kBlock, 0, s>>>(A->row_val64.p, A->row_val32.p, nnz);
+        FL_LAUNCH_CHECK();
+    }
+    FL_HIP(hipStreamSynchronize(s));
+    A->row_ready = true;
+    return FL_OK;
+}
+
 // Vector of VEC consecutive T (VEC = 1, 2 or 4), loaded and stored as one access.
 template <typename T, int VEC>
 struct RowVec;

@@ -54,6 +54,7 @@ struct fl_matrix {
     //   reuses the front of the product array and a chunk's products can stay in the Infinity
     //   Cache between its two phases.
     int pb_R = 0, nbands = 0, nruns = 0, pb_chunks = 1;
+    long long pb_ring = 0;            // product-array entries the chunks need (the largest chunk)
     bool pb_ready = false;
     std::vector<int> pb_chunk_batch;  // [pb_chunks+1] first batch of every chunk
     std::vector<int> pb_chunk_off;    // [pb_chunks] product-array offset (a multiple of 4)
@@ -92,7 +93,11 @@ void colsort_device(int n, const int *d_counts, int *d_perm, int *d_sortnnz, int
 // Source of element (sorted column i, depth d):
 //   tmp mode (d_P != nullptr): tmp_row/tmp_val[P[i] + d]
 //   fused mode: rowidx/val[colptr[perm[i]] + d]
+//   tiled mode (tiled = true): A->sortrowidx / A->val64 already hold the elements in tile order
+//     (a layout read back from a file); only the level table, tile metadata and classification
+//     are computed
 struct ElemSource {
+    bool tiled = false;
     const int *P = nullptr;
     const int *colptr = nullptr;
     const int *perm = nullptr;
@@ -101,5 +106,11 @@ struct ElemSource {
 };
 void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src, hipStream_t s, float *t_tran_ms,
                       float *t_clas_ms);
+
+// y = A x (assign) or y += A x with one schedule. work: 2*m floats for the FAST kernel's split
+// BF16 mode. pb_bins: product array for the PB schedule (at least pb_ring entries of the value
+// kind's product type, 16-byte aligned); nullptr = the matrix's own, allocated on first use.
+int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s,
+         void *pb_bins = nullptr);
 
 }  // namespace fl

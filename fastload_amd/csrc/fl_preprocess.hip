@@ -385,18 +385,22 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
     A->tile_colidx.alloc(std::max(T, 1));
     A->tile_format.alloc(std::max(T, 1));
     DevBuf<int> tile_depth(std::max(T, 1));
-    A->sortrowidx.alloc(std::max(nnz, 1));
+    if (!src.tiled) {
+        A->sortrowidx.alloc(std::max(nnz, 1));
+        A->val64.alloc(std::max(nnz, 1));
+    }
     A->sortrowindex.alloc(std::max(nnz, 1));
-    A->val64.alloc(std::max(nnz, 1));
     if (T > 0) {
         k_tiles<<<stride_grid(T), kBlock, 0, s>>>(T, L, n, nnz, W, A->lvl_col.p, A->lvl_base.p, A->lvl_tall.p,
                                                   A->lvl_tileoff.p, A->lvl_nnzoff.p, A->tile_ptr.p, A->tile_len.p,
                                                   A->tile_colidx.p, tile_depth.p);
         FL_LAUNCH_CHECK();
-        k_tile_data<<<stride_grid((long long)T * W), kBlock, 0, s>>>(T, W, A->tile_ptr.p, A->tile_len.p,
-                                                                     A->tile_colidx.p, tile_depth.p, src,
-                                                                     A->sortrowidx.p, A->val64.p);
-        FL_LAUNCH_CHECK();
+        if (!src.tiled) {
+            k_tile_data<<<stride_grid((long long)T * W), kBlock, 0, s>>>(T, W, A->tile_ptr.p, A->tile_len.p,
+                                                                         A->tile_colidx.p, tile_depth.p, src,
+                                                                         A->sortrowidx.p, A->val64.p);
+            FL_LAUNCH_CHECK();
+        }
     } else {
         FL_HIP(hipMemsetAsync(A->tile_ptr.p, 0, sizeof(int), s));
     }

@@ -239,8 +239,8 @@ __global__ void k_check_win(const int *__restrict__ batch_of, const uint16_t *__
             atomicOr(flag, 1);
             continue;
         }
-        const int r = win_rows[win_off[b] + slot[g]];
-        if (r < 0 || r >= m) atomicOr(flag, 2);
+        const int r = win_rows[win_off[b] + slot[g]] & 0x7fffffff;  // high bit: row shared with other batches
+        if (r >= m) atomicOr(flag, 2);
     }
 }
 
@@ -416,6 +416,7 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
         FL_HIP(hipMemsetAsync(A->pb_band_start.p, 0, sizeof(int) * ((size_t)nbands + 1), s));
         FL_HIP(hipMemsetAsync(A->pb_run_off.p, 0, sizeof(int) * ((size_t)nb + 1), s));
         A->nruns = 0;
+        A->pb_ring = 0;
         A->pb_ready = true;
         return;
     }
@@ -491,9 +492,117 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
         throw HipError{hipErrorInvalidValue};
     }
     A->pb_chunks = C;
+    A->pb_ring = ring;
     A->pb_chunk_batch = cb;
     A->pb_chunk_off = off;
     A->pb_ready = true;
+}
+
+// ----------------------------------------------------------- schedules read back from a file
+// The arrays come from host memory (a file); every index a kernel will use is checked before
+// the schedule is marked ready, on the host where the check is a scan and on the device with
+// the validators above, so a damaged file is refused instead of indexing out of bounds.
+template <typename T>
+static void upload_to(DevBuf<T> &d, const T *h, size_t count, hipStream_t s) {
+    d.alloc(std::max<size_t>(count, 1));
+    if (count) FL_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+static bool ptr_ok(const int *p, size_t count, int first, int last) {
+    if (p[0] != first || p[count - 1] != last) return false;
+    for (size_t i = 0; i + 1 < count; ++i)
+        if (p[i] > p[i + 1]) return false;
+    return true;
+}
+
+static int bad_file(const char *what) {
+    set_error("fl_matrix_load: %s", what);
+    return FL_ERR_INVALID;
+}
+
+int win_install(fl_matrix *A, int bt, bool rows, int nb_file, const uint16_t *slot, const int *win_rows, long long D,
+                const int *off, hipStream_t s) {
+    ensure_batches(A, bt, rows ? 1 : 0, s);
+    const int nb = A->nbatches, nnz = A->nnz;
+    if (nb != nb_file) return bad_file("WIN schedule: batch count differs from the layout's");
+    if (D < 0 || D > (long long)nnz || !ptr_ok(off, (size_t)nb + 1, 0, (int)D))
+        return bad_file("WIN schedule: slot offsets are not a pointer array over the distinct rows");
+    int maxslots = 0;
+    for (int b = 0; b < nb; ++b) maxslots = std::max(maxslots, off[b + 1] - off[b]);
+    if (maxslots > bt * 64) return bad_file("WIN schedule: a batch has more slots than entries");
+    upload_to(A->win_slot, slot, nnz, s);
+    upload_to(A->win_rows, win_rows, (size_t)D, s);
+    upload_to(A->win_off, off, (size_t)nb + 1, s);
+    DevBuf<int> flag(1);
+    FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    if (nnz)
+        k_check_win<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->win_slot.p, A->win_off.p, bt * 64, nnz, A->m,
+                                               A->win_rows.p, flag.p);
+    FL_LAUNCH_CHECK();
+    if (read1(flag.p, s)) {
+        A->win_slot.release();
+        A->win_rows.release();
+        A->win_off.release();
+        return bad_file("WIN schedule: slots or rows out of range");
+    }
+    A->win_maxslots = maxslots;
+    A->win_ready = true;
+    return FL_OK;
+}
+
+int pb_install(fl_matrix *A, int bt, int R, int nb_file, int C, const int *chunk_batch, const int *chunk_off,
+               long long ring, const uint16_t *lslot, const uint16_t *rowlocal, const int *band_start,
+               const int *run_off, long long nruns, const int *run_lstart, const int *run_dest, const int *run_len,
+               hipStream_t s) {
+    ensure_batches(A, bt, 0, s);
+    const int nb = A->nbatches, nnz = A->nnz, m = A->m;
+    if (nb != nb_file) return bad_file("PB schedule: batch count differs from the layout's");
+    if (R < 1024 || R > 16384) return bad_file("PB schedule: band rows outside [1024, 16384]");
+    const int nbands = std::max((m + R - 1) / R, 1);
+    if (C < 1 || (long long)C * nbands > 0x7fffffffLL) return bad_file("PB schedule: bad chunk count");
+    if (ring < 0 || ring > nnz || nruns < 0 || nruns > (long long)nnz)
+        return bad_file("PB schedule: product ring or run count out of range");
+    if (chunk_batch[0] != 0 || chunk_batch[C] != nb) return bad_file("PB schedule: chunks do not cover the batches");
+    for (int c = 0; c < C; ++c)
+        if (chunk_batch[c] >= chunk_batch[c + 1] || (c > 0 && chunk_batch[c] % 8))
+            return bad_file("PB schedule: chunk boundaries not increasing multiples of 8");
+    const size_t nbs = (size_t)C * nbands + 1;
+    if (!ptr_ok(band_start, nbs, 0, nnz)) return bad_file("PB schedule: band starts are not a pointer array over nnz");
+    for (int c = 0; c < C; ++c) {
+        const long long lo = band_start[(size_t)c * nbands], hi = band_start[(size_t)(c + 1) * nbands];
+        if (chunk_off[c] < 0 || chunk_off[c] % 4 || chunk_off[c] > lo || hi - chunk_off[c] > ring)
+            return bad_file("PB schedule: a chunk's products fall outside the product ring");
+    }
+    if (!ptr_ok(run_off, (size_t)nb + 1, 0, (int)nruns)) return bad_file("PB schedule: run offsets invalid");
+    for (long long i = 0; i < nnz; ++i)
+        if (rowlocal[i] >= R) return bad_file("PB schedule: a band-local row exceeds the band");
+    upload_to(A->pb_lslot, lslot, nnz, s);
+    upload_to(A->pb_rowlocal, rowlocal, nnz, s);
+    upload_to(A->pb_band_start, band_start, nbs, s);
+    upload_to(A->pb_run_off, run_off, (size_t)nb + 1, s);
+    upload_to(A->pb_run_lstart, run_lstart, (size_t)nruns, s);
+    upload_to(A->pb_run_dest, run_dest, (size_t)nruns, s);
+    upload_to(A->pb_run_len, run_len, (size_t)nruns, s);
+    DevBuf<int> flag(1);
+    FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    if (nnz)
+        k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->batch_start.p, bt * 64, nnz, (int)ring,
+                                              A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p,
+                                              nb, flag.p);
+    FL_LAUNCH_CHECK();
+    if (read1(flag.p, s)) {
+        A->pb_ready = false;
+        return bad_file("PB schedule: staging slots or runs out of range");
+    }
+    A->pb_R = R;
+    A->nbands = nbands;
+    A->nruns = (int)nruns;
+    A->pb_chunks = C;
+    A->pb_ring = ring;
+    A->pb_chunk_batch.assign(chunk_batch, chunk_batch + C + 1);
+    A->pb_chunk_off.assign(chunk_off, chunk_off + C);
+    A->pb_ready = true;
+    return FL_OK;
 }
 
 // ------------------------------------------------------------------------------ kernels
@@ -861,29 +970,36 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
     }
 }
 
-// The caller (spmv) has checked that the schedule and the value copy are prepared.
-int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s) {
+// The caller (spmv) has checked that the schedule and the value copy are prepared. PB writes
+// its products to pb_bins when given (the caller guarantees pb_ring entries), else to the
+// matrix's own product array, allocated on first use.
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
+                 void *pb_bins) {
     const bool win = (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR);
-    if (!win) {
+    double *bins64 = (double *)pb_bins;
+    float *bins32 = (float *)pb_bins;
+    if (!win && !pb_bins) {
         if (kind == FL_VAL_F64) { if (!A->pb_bins64.p) A->pb_bins64.alloc(std::max(A->nnz, 1)); }
         else if (!A->pb_bins32.p) A->pb_bins32.alloc(std::max(A->nnz, 1));
+        bins64 = A->pb_bins64.p;
+        bins32 = A->pb_bins32.p;
     }
     switch (kind) {
         case FL_VAL_F64:
             if (win) launch_win<double, double, double, false>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
-            else launch_pb<double, double, double, double, false>(A, A->val64.p, (const double *)x, A->pb_bins64.p, (double *)y, assign, s);
+            else launch_pb<double, double, double, double, false>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s);
             return FL_OK;
         case FL_VAL_F32:
             if (win) launch_win<float, float, float, false>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<float, float, float, float, false>(A, A->val32.p, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
+            else launch_pb<float, float, float, float, false>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16:
             if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, (const float *)x, A->pb_bins32.p, (float *)y, assign, s);
+            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16X2:
             if (win) launch_win<uint32_t, float, double, true>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
-            else launch_pb<uint32_t, float, float, double, true>(A, A->bf16_pair.p, (const float *)x, A->pb_bins32.p, (double *)y, assign, s);
+            else launch_pb<uint32_t, float, float, double, true>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);

@@ -255,7 +255,8 @@ static void launch_fast(const fl_matrix *A, const V *val, const V *val_lo, const
     FL_LAUNCH_CHECK();
 }
 
-int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s);
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
+                 void *pb_bins);
 void zero_fill(void *d, size_t bytes, hipStream_t s);
 
 static int refuse(const char *msg) {
@@ -307,14 +308,15 @@ static int spmv_check(const fl_matrix *A, int kernel, int kind, const void *work
 // assign: y = A x instead of y += A x. The PB schedule writes every row once and needs no
 // clearing; the other kernels accumulate, so y is cleared with a fill kernel first (WIN and
 // WINR then store the rows a single batch owns and add the shared ones atomically).
-int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s) {
+int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s,
+         void *pb_bins) {
     const int rc = spmv_check(A, kernel, kind, work);
     if (rc) return rc;
     const size_t ybytes = (size_t)A->m * ((kind == FL_VAL_F64 || kind == FL_VAL_BF16X2) ? 8 : 4);
     if (assign && (A->tilenum == 0 || kernel != FL_KERNEL_PB)) zero_fill(y, ybytes, s);
     if (A->tilenum == 0) return FL_OK;
     if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR || kernel == FL_KERNEL_PB)
-        return spmv_batched(A, kernel, kind, x, y, assign, s);
+        return spmv_batched(A, kernel, kind, x, y, assign, s, pb_bins);
     if (kernel == FL_KERNEL_DROPIN) {
         if (kind == FL_VAL_F64) {
             if (A->W == 32) launch_dropin<double, 32>(A, (const double *)x, (double *)y, s);

@@ -127,6 +127,23 @@ int fl_FastLoad_cpu_f32(const slide_matrix *matrix, const float *sortval, int nn
  * (NULL fields are skipped) and zeroes the struct. */
 void fl_Tile_destroy(slide_matrix *matrix);
 
+/* ------------------------------------------------- on-disk cache of the preprocessing
+ * No reference counterpart: the reference rebuilds the layout on every run (src/main.cu:151-248).
+ * A versioned binary file ("FLMATRIX", version 1, little-endian) holds the tiled layout -- the
+ * slide_matrix fields of src/format.h:6-20 and the column order -- and, when written from a
+ * device-resident matrix, every prepared schedule. Every section carries a 64-bit hash; a reader
+ * refuses a file whose magic, version, byte order, section bounds, lengths or hashes do not match
+ * (FL_ERR_INVALID; FL_ERR_OPEN when the file cannot be opened, read or written).
+ *
+ * fl_slide_save writes a host slide_matrix (as fl_formatTransform returns it) with its sizes, the
+ * level count and, when perm is not NULL, the column order (colA entries: sortx = x[perm]).
+ * fl_slide_load reads the slide_matrix sections of any such file (also one written by
+ * fl_matrix_save) into malloc'd arrays (free with fl_Tile_destroy; *perm, if asked for and
+ * present, with fl_free; else set to NULL). */
+int fl_slide_save(const char *path, const slide_matrix *matrix, int rowA, int colA, int nnz, int h_count,
+                  const int *perm);
+int fl_slide_load(const char *path, slide_matrix *matrix, int *rowA, int *colA, int *nnz, int *h_count, int **perm);
+
 /* ------------------------------------------------------------ device memory (plumbing)
  * Minimal device-memory helpers so callers without a GPU framework can keep inputs resident.
  * Streams are passed as void* (a hipStream_t, or NULL for the default stream). */
@@ -233,6 +250,15 @@ int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d
 int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work,
                    void *stream);
 
+/* fl_matrix_save writes a device-resident matrix: its layout (width, column order, tiles,
+ * elements, classification), which value copies were prepared, and the prepared WIN/WINR, PB and
+ * row-order schedules. fl_matrix_load reads one back into a new handle: every row and column
+ * index, pointer array and schedule slot or run is checked before use; the level table, tile
+ * metadata, classification, batch membership and value copies are recomputed on the device (the
+ * stored tile metadata must agree with them); the schedule sorts are not redone. */
+int fl_matrix_save(const fl_matrix *A, const char *path, void *stream);
+int fl_matrix_load(fl_matrix **A, const char *path, void *stream);
+
 /* Builds (once per parameter set) the preprocessing of a batched schedule on a 64-wide matrix:
  *   kernel FL_KERNEL_WIN / FL_KERNEL_WINR: batch_tiles in {64, 128, 256} (LDS <= batch_tiles*512 bytes);
  *   kernel FL_KERNEL_PB : batch_tiles as above, band_rows in [1024, 16384] (LDS = band_rows*8 bytes).
@@ -250,6 +276,23 @@ int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int ba
  * atomics. Results do not depend on the lane width or on the other rows of the matrix. */
 int fl_matrix_prepare_rows(fl_matrix *A, void *stream);
 int fl_spmm(fl_matrix *A, int val_kind, int k, const void *d_X, void *d_Y, int accumulate, void *stream);
+
+/* ------------------------------------------ iterative solver (no reference counterpart)
+ * Conjugate gradients for A x = b on a square, symmetric positive definite device-resident matrix
+ * -- the caller that repeats the SpMV (the reference repeats it only to time it,
+ * src/FastLoad_GPU.h:349-374). The product uses one prepared schedule: kernel as for fl_spmv, or
+ * FL_KERNEL_ROW (the row-ordered kernel, after fl_matrix_prepare_rows); val_kind FL_VAL_F64
+ * (double b, x and vectors) or FL_VAL_F32 (float; dot products summed in FP64). A must outlive the
+ * solver. fl_cg_solve starts from the x it is given, runs iterations in groups of check_every
+ * (rounded up to even) captured once as a hipGraph, and after each group stops when
+ * ||r|| / ||b|| <= rtol or max_iter iterations have run: *iters receives the iterations run (a
+ * multiple of the group), *rel_res the final ||r|| / ||b|| (recursive residual). */
+typedef struct fl_cg fl_cg;
+#define FL_KERNEL_ROW (-1)
+int fl_cg_create(fl_cg **cg, fl_matrix *A, int kernel, int val_kind);
+int fl_cg_solve(fl_cg *cg, const void *d_b, void *d_x, int max_iter, double rtol, int check_every, int *iters,
+                double *rel_res);
+int fl_cg_destroy(fl_cg *cg);
 
 /* --------------------------------------------- batches of independent matrices
  * Block-diagonal assembly A = diag(A_0, ..., A_{count-1}) of host CSC matrices: one SpMV/SpMM
@@ -310,7 +353,6 @@ int fl_mlp_train_step(fl_mlp *P, const float *d_X, const float *d_T, int batch, 
  * (fl_spmv; FL_KERNEL_ROW = fl_spmm with k = 1 after fl_matrix_prepare_rows), batch tiles and
  * band rows (fl_matrix_prepare_schedule). Class 0 is the reference schedule. */
 #define FL_SCHEDULES 12
-#define FL_KERNEL_ROW (-1)
 int fl_schedule_info(int cls, int *width, int *kernel, int *batch_tiles, int *band_rows);
 /* fl_features + fl_mlp_predict on a device CSC matrix: the network's class and its probability.
  * Falls back to class 0 (probability 0) when net is NULL, the matrix has no non-zeros or a

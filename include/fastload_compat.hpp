@@ -9,6 +9,7 @@
 #include <cstdio>
 
 #include "fastload.h"
+#include "fastload_compare.h"
 
 namespace fastload_compat_detail {
 inline void report(int rc, const char *what) {
@@ -64,5 +65,23 @@ inline void FastLoad_cpu(slide_matrix *matrix, int nnz, int rowA, int colA, doub
 
 // src/format.h:29-42
 inline void Tile_destroy(slide_matrix *matrix) { fl_Tile_destroy(matrix); }
+
+// The two comparators src/main.cu times beside FastLoad (measurement only; a caller of these two
+// links libfastload_compare.so as well).
+// src/CSCSpMV.h:29-40
+inline void csc_spmv(char *filename, int m, int n, int nnz, int *csc_ptr, int *csc_rowidx, double *csc_val, double *x,
+                     double *y, double *y_golden) {
+    const int rc = flc_csc_spmv(filename, m, n, nnz, csc_ptr, csc_rowidx, csc_val, x, y, y_golden, nullptr);
+    if (rc < 0) std::fprintf(stderr, "FastLoad csc_spmv failed (%d): %s\n", rc, flc_last_error());
+}
+
+// src/cusparse_cuda.h:27-36 (rocSPARSE in place of cuSPARSE)
+inline void call_cusparse_spmv(char *filename, const int A_nnz, const int A_num_rows, const int A_num_cols,
+                               int *hA_csrOffsets, int *hA_columns, double *hA_values, double *hX, double *hY,
+                               double *hY_result) {
+    const int rc = flc_rocsparse_spmv(filename, A_nnz, A_num_rows, A_num_cols, hA_csrOffsets, hA_columns, hA_values,
+                                      hX, hY, hY_result, nullptr);
+    if (rc < 0) std::fprintf(stderr, "FastLoad call_cusparse_spmv failed (%d): %s\n", rc, flc_last_error());
+}
 
 #endif  // FASTLOAD_COMPAT_HPP

@@ -1,7 +1,7 @@
 // A driver with the call sequence of the reference src/main.cu:14-322 (loader, csr_to_csc, ColSort,
-// formatTransform, FastLoad_cpu, FastLoad_spmv, Tile_destroy), compiled against
-// include/fastload_compat.hpp and linked with libfastload.so by
-// tests/test_gpu_parity.py::test_reference_style_driver.
+// formatTransform, FastLoad_cpu, the naive CSC and vendor comparators, FastLoad_spmv, Tile_destroy),
+// compiled against include/fastload_compat.hpp and linked with libfastload.so and
+// libfastload_compare.so by tests/test_gpu_parity.py::test_reference_style_driver.
 #include <cstdlib>
 #include <cstring>
 #include "fastload_compat.hpp"
@@ -21,6 +21,10 @@ int main(int argc, char **argv) {
   double *y = (double*)calloc(rowA, 8), *g = (double*)calloc(rowA, 8);
   for (int c = 0; c < colA; c++) for (int j = cp[c]; j < cp[c+1]; j++) g[cr[j]] += cv[j] * x[c];
   FastLoad_cpu(&A, nnz, rowA, colA, sx, y, g);  // src/main.cu:263-277 (prints only on a mismatch)
+  std::memset(y, 0, 8 * (size_t)rowA);
+  csc_spmv(argv[1], rowA, colA, nnz, cp, cr, cv, x, y, g);  // src/main.cu:288-298
+  std::memset(y, 0, 8 * (size_t)rowA);
+  call_cusparse_spmv(argv[1], nnz, rowA, colA, rp, ci, v, x, y, g);  // src/main.cu:300-310
   std::memset(y, 0, 8 * (size_t)rowA);
   FastLoad_spmv(argv[1], &A, nnz, colA, rowA, sx, y, g);
   Tile_destroy(&A);

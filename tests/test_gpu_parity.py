@@ -193,12 +193,14 @@ def test_reference_style_driver(gpu, tmp_path):
     from conftest import ROOT
     exe = tmp_path / "drop_in_main"
     subprocess.run(["g++", "-std=c++17", "-I", str(ROOT / "include"), str(ROOT / "tests" / "drop_in_main.cpp"),
-                    "-L", str(ROOT / "fastload_amd"), "-lfastload", f"-Wl,-rpath,{ROOT / 'fastload_amd'}", "-o", str(exe)],
+                    "-L", str(ROOT / "fastload_amd"), "-lfastload", "-lfastload_compare",
+                    f"-Wl,-rpath,{ROOT / 'fastload_amd'}", "-o", str(exe)],
                    check=True)
     r = subprocess.run([str(exe), str(GOLDEN / "1138_bus.mtx")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "Check FASTLOAD GPU PASS!" in r.stdout, r.stdout + r.stderr
     assert "Check NO PASS" not in r.stdout, r.stdout  # FastLoad_cpu and FastLoad_spmv both exact
+    assert "BASIC CSC GPU PASS!" in r.stdout and "rocSPARSE test PASSED!" in r.stdout, r.stdout
 
 
 @pytest.mark.parametrize("kernel,kind,W", [(fl.KERNEL_DROPIN, fl.VAL_F64, 32), (fl.KERNEL_FAST, fl.VAL_F32, 64),

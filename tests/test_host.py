@@ -33,6 +33,21 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(fl.lib, name)
 
 
+def test_compare_library_exports_its_header():
+    """libfastload_compare.so (measurement-only comparators) exports every flc_ symbol of
+    include/fastload_compare.h, and libfastload.so does not depend on it."""
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "fastload_compare.h").read_text(), flags=re.S)
+    names = {mt.group(1) for mt in re.finditer(r"\b(flc_[A-Za-z0-9_]+)\s*\(", text)}
+    assert "flc_csc_spmv" in names and "flc_rocsparse_run" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", str(ROOT / "fastload_amd" / "libfastload_compare.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert not names - exported, names - exported
+    deps = subprocess.run(["readelf", "-d", str(ROOT / "fastload_amd" / "libfastload.so")], capture_output=True,
+                          text=True, check=True).stdout
+    assert "rocsparse" not in deps and "fastload_compare" not in deps
+
+
 def test_compat_header_compiles():
     src = ROOT / "include" / "fastload_compat.hpp"
     if not src.exists():
@@ -322,4 +337,90 @@ def test_wrappers_refuse_bad_buffers(call):
     and the value kind before any pointer reaches the library."""
     with pytest.raises(fl.FastLoadError) as e:
         call(_fake_matrix(7, 10))
+    assert e.value.rc == -8
+
+
+# ------------------------------------------------------------- slide_matrix file (host)
+@pytest.mark.parametrize("name,W", [("ref_layout_1138_bus", 32), ("ref_layout_1138_bus", 64), ("ref_layout_symgeom", 32),
+                                    ("ref_layout_symgeom", 64)])
+def test_slide_file_round_trip(tmp_path, name, W):
+    """fl_slide_save -> fl_slide_load gives back every slide_matrix array bit for bit, and the loaded
+    layout multiplies exactly as the original (FastLoad_cpu, serial)."""
+    g = np.load(GOLDEN / f"{name}.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    t = _golden_tiles(g, W)
+    p = tmp_path / "a.flmx"
+    fl.slide_save(p, t, m, n, nnz, 14, perm=g["perm"])
+    t2, m2, n2, nnz2, h2, perm2 = fl.slide_load(p)
+    assert (m2, n2, nnz2, h2) == (m, n, nnz, 14)
+    assert np.array_equal(perm2, g["perm"])
+    for k in ("tile_ptr", "tile_len", "tile_colidx", "tile_format", "sortrowidx", "sortrowindex", "countrow",
+              "segmentoffset"):
+        assert np.array_equal(getattr(t2, k), getattr(t, k)), k
+    assert np.array_equal(t2.sortval.view(np.uint64), t.sortval.view(np.uint64))
+    assert (t2.tilenum, t2.segsum) == (t.tilenum, t.segsum)
+    y1, _ = fl.FastLoad_cpu(t, nnz, m, n, g["sortx"])
+    y2, _ = fl.FastLoad_cpu(t2, nnz, m, n, g["x"][perm2])
+    assert np.array_equal(y1.view(np.uint64), y2.view(np.uint64))
+
+
+def test_slide_file_format_and_damage(tmp_path):
+    """The header starts with "FLMATRIX", version 1 and the byte-order mark 0x01020304; a file whose
+    magic, version, length or section contents were changed is refused, a missing file gives
+    FL_ERR_OPEN, and a file without a column order loads with perm None."""
+    import struct
+    g = np.load(GOLDEN / "ref_layout_1138_bus.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    t = _golden_tiles(g, 32)
+    p = tmp_path / "b.flmx"
+    fl.slide_save(p, t, m, n, nnz, 14)
+    raw = bytearray(p.read_bytes())
+    assert raw[:8] == b"FLMATRIX" and struct.unpack_from("<II", raw, 8) == (1, 0x01020304)
+    assert struct.unpack_from("<iii", raw, 16) == (m, n, nnz)
+    assert fl.slide_load(p)[5] is None
+
+    def refused(data, rc=-8):
+        q = tmp_path / "bad.flmx"
+        q.write_bytes(bytes(data))
+        with pytest.raises(fl.FastLoadError) as e:
+            fl.slide_load(q)
+        assert e.value.rc == rc
+
+    bad = bytearray(raw); bad[0] ^= 1; refused(bad)                     # magic
+    bad = bytearray(raw); bad[8] = 2; refused(bad)                      # version
+    refused(raw[: len(raw) - 100])                                      # truncated: a section leaves the file
+    bad = bytearray(raw); bad[-5] ^= 0x40; refused(bad)                 # last section's data: hash mismatch
+    with pytest.raises(fl.FastLoadError) as e:
+        fl.slide_load(tmp_path / "missing.flmx")
+    assert e.value.rc == -1
+
+
+def test_slide_file_empty_matrix(tmp_path):
+    t = fl.Tiles(0, 0, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                 np.zeros(0, np.int32), np.zeros(0), np.zeros(0, np.int32), np.zeros(1, np.int32), np.zeros(1, np.int32))
+    p = tmp_path / "e.flmx"
+    fl.slide_save(p, t, 3, 4, 0, 0, perm=np.arange(4, dtype=np.int32))
+    t2, m, n, nnz, h, perm = fl.slide_load(p)
+    assert (m, n, nnz, t2.tilenum) == (3, 4, 0, 0) and perm.tolist() == [0, 1, 2, 3]
+
+
+def test_slide_file_hashes_and_tamper_detection(tmp_path):
+    """The test-side restatement of the section hash agrees with the library's for every section;
+    a section changed and re-hashed still loads (the hash guards accidents; the host API validates
+    layouts when they are used, as FastLoad_cpu's refusal shows)."""
+    from flmx_util import section_hash, sections, tamper
+    g = np.load(GOLDEN / "ref_layout_1138_bus.npz")
+    m, n, nnz = int(g["m"]), int(g["n"]), int(g["nnz"])
+    p = tmp_path / "h.flmx"
+    fl.slide_save(p, _golden_tiles(g, 32), m, n, nnz, 14, perm=g["perm"])
+    raw = p.read_bytes()
+    secs = sections(raw)
+    assert set(secs) == {1, 3, 4, 5, 6, 7, 8, 9, 10, 11}
+    for tag, (_, elem, count, off, h) in secs.items():
+        assert section_hash(raw[off:off + elem * count]) == h, tag
+    q = tmp_path / "t.flmx"
+    q.write_bytes(bytes(tamper(raw, 4, 0, 31, "i")))  # tile 0's length no longer matches its elements
+    t2, *_ = fl.slide_load(q)
+    with pytest.raises(fl.FastLoadError) as e:
+        fl.FastLoad_cpu(t2, nnz, m, n, g["sortx"])
     assert e.value.rc == -8
