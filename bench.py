@@ -30,6 +30,7 @@ Algorithmic bytes per SpMV: B = nnz*(s_val + 4) + n*s_x + m*s_y.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -75,13 +76,16 @@ VARIANTS = {
     "pb4k_f32": (64, 3, 1, 4, 256, 4096),
     "pb128_8k_f32": (64, 3, 1, 4, 128, 8192),
 }
-KIND_SVAL = {0: ("f64", 8), 1: ("f32", 4), 2: ("bf16", 2), 3: ("bf16x2", 4)}
+# value storage -> (name, value bytes beside a 4-byte row index); packed words hold the value and
+# the schedule's slot in 4 bytes, so they count 0 + 4
+KIND_SVAL = {0: ("f64", 8), 1: ("f32", 4), 2: ("bf16", 2), 3: ("bf16x2", 4), 4: ("packed", 0)}
 
 
-def learned_variants(sched, names=("f32", "bf16x2", "bf16")):
-    """Value-storage variants of the schedule the selector chose: FP32, split hi+lo BF16 and BF16
-    (those of `names` that the schedule supports without a work buffer)."""
-    kinds = [1, 3, 2] if sched.kernel in (2, 3, 4) else ([1, 2] if sched.kernel >= 0 else [1])
+def learned_variants(sched, names=("f32", "bf16x2", "bf16", "packed")):
+    """Value-storage variants of the schedule the selector chose: FP32, split hi+lo BF16, BF16 and
+    packed value + slot words (those of `names` that the schedule supports without a work
+    buffer)."""
+    kinds = [1, 3, 2, 4] if sched.kernel in (2, 3, 4) else ([1, 2] if sched.kernel >= 0 else [1])
     return {f"learned_{KIND_SVAL[k][0]}": (sched.width, sched.kernel, k, KIND_SVAL[k][1], sched.batch_tiles,
                                            sched.band_rows) for k in kinds if KIND_SVAL[k][0] in names}
 
@@ -224,6 +228,9 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup, flush
         if flushed is not None:
             res[vname].update(kernel_ms_flushed=round(flushed, 5), kernel_gbs_flushed=round(B / flushed / 1e6, 2),
                               kernel_gbs_equiv_f32_flushed=round(B32 / flushed / 1e6, 2))
+        if kind == fl.VAL_PACKED:
+            mb, eb, ib = M.pack_info(kernel)
+            res[vname].update(pack_mantissa_bits=mb, pack_exponent_bits=eb, pack_slot_bits=ib)
         log(f"[{name}] {vname:14s} step {step_ms:.4f} ms kernel {kern_ms:.4f} ms  {res[vname]['gbs']:7.1f} GB/s "
             f"(f32-equiv {res[vname]['gbs_equiv_f32']:7.1f})  err {err:.2e}"
             + (f"  flushed kernel {flushed:.4f} ms" if flushed is not None else ""))
@@ -271,7 +278,7 @@ def run_comparators(fl, name, m, n, dcp, dri, dv, x, y_ref, steps, warmup, flush
     return res
 
 
-def run_batched(fl, count, seed, steps, warmup):
+def run_batched(fl, count, seed, steps, warmup, sequential=True, ks=(1, 8, 32)):
     """Batched configuration: `count` square CSC matrices, n log-uniform in [1e3, 2e5], column nnz
     Zipf(2.0) truncated at n, FP32, k = 1, 8, 32 right-hand sides. The batch runs as one
     block-diagonal matrix (k = 1: the schedule the learned selector picks for it; k > 1: the
@@ -322,7 +329,7 @@ def run_batched(fl, count, seed, steps, warmup):
     # per-matrix objects for the sequential baselines: the 32-wide drop-in layout (plain) and the
     # learned schedule's layout (same)
     singles = []
-    for (m, n, c, r, vv) in mats:
+    for (m, n, c, r, vv) in (mats if sequential else []):
         P = fl.FastLoadMatrix(m, n, c, r, vv, width=32)
         P.prepare(fl.VAL_F32)
         Q = fl.FastLoadMatrix(m, n, c, r, vv, width=64)
@@ -334,7 +341,7 @@ def run_batched(fl, count, seed, steps, warmup):
             S.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
         singles.append((P, Q, S, P.perm(), S.perm() if sched.kernel >= 0 else None))
     log(f"[batched] per-matrix preprocessing done ({time.time() - t0:.1f}s)")
-    for k in (1, 8, 32):
+    for k in ks:
         X = rng.standard_normal((N_, k)).astype(np.float32)
         dX = DeviceArray.from_host(X.ravel())
         dY = DeviceArray(M_ * k, np.float32)
@@ -356,6 +363,12 @@ def run_batched(fl, count, seed, steps, warmup):
             batch_step()
         t_graph = timed(g.launch, steps)
         t_eager = timed(batch_step, steps)
+        if not sequential:
+            res[f"k{k}"] = dict(graph_ms=round(t_graph, 4), eager_ms=round(t_eager, 4),
+                                graph_gbs=round(B_alg / t_graph / 1e6, 1), bytes_alg=B_alg)
+            log(f"[batched] k={k:2d} graph {t_graph:.3f} ms  eager {t_eager:.3f} ms  ({res[f'k{k}']['graph_gbs']} GB/s)")
+            del g
+            continue
         # plain drop-in, one launch per matrix and right-hand side: x columns permuted on the host,
         # matrix i's column j at offset co[i] * k + j * n_i (likewise y)
         XP = np.empty(N_ * k, np.float32)
@@ -574,89 +587,69 @@ def run_c1(fl, steps, warmup):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--n", type=int, default=4194304)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-stencil", action="store_true")
-    ap.add_argument("--no-c1", action="store_true")
-    ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timings")
-    ap.add_argument("--no-compare", action="store_true", help="skip the naive-CSC and rocSPARSE yardsticks")
-    ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
-    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
-    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,winr64_f32,pb_f32,row_f32")
-    ap.add_argument("--schedule", default=None, help="use this schedule for the learned variants instead of "
-                    "selecting one (profiling runs)")
-    ap.add_argument("--stencil-schedule", default=None, help="the same for the stencil workload")
-    ap.add_argument("--learned-kinds", default="f32,bf16x2,bf16",
-                    help="value storages timed for the learned schedule")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this FastLoad build runs on one GPU only (no multi-GPU path is built); "
-                                   "run with --gpus 1"}))
-        sys.exit(2)
-
+def run_all(args):
+    """Every workload bench.py times; returns the JSON object."""
     import fastload_amd as fl
     from fastload_amd import DeviceArray
     from fastload_amd import workloads as wl
 
-    t0 = time.time()
-    m, n, cp, ri, v, x = wl.powerlaw(args.n, args.seed)
-    log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
-    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
-    del cp, ri, v
-    sched, chosen = select_schedule(fl, m, n, dcp, dri, dv, force=args.schedule)
-    log(f"[powerlaw] selector chose {chosen}")
-    variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
-    variants.update(learned_variants(sched, args.learned_kinds.split(",")))
     flush = None if args.no_flush else DeviceArray(1 << 28, np.float32)  # 1 GiB, 4x the Infinity Cache
-    nnz, mats, res, y_ref = run_workload(fl, "powerlaw", m, n, dcp, dri, dv, x, variants, args.steps, args.warmup,
-                                         flush)
-    del mats
-    comparators = {} if args.no_compare else {
-        "powerlaw": run_comparators(fl, "powerlaw", m, n, dcp, dri, dv, x, y_ref, args.steps, args.warmup, flush)}
-    del dcp, dri, dv
+    out, comparators = {"profiling_run": "the power-law workload was skipped (--no-powerlaw)"}, {}
+    if not args.no_powerlaw:
+        t0 = time.time()
+        m, n, cp, ri, v, x = wl.powerlaw(args.n, args.seed)
+        log(f"[powerlaw] generated {m}x{n} nnz={int(cp[n])} in {time.time() - t0:.1f}s")
+        dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
+        del cp, ri, v
+        sched, chosen = select_schedule(fl, m, n, dcp, dri, dv, force=args.schedule)
+        log(f"[powerlaw] selector chose {chosen}")
+        variants = {k: VARIANTS[k] for k in args.variants.split(",") if k}
+        variants.update(learned_variants(sched, args.learned_kinds.split(",")))
+        nnz, mats, res, y_ref = run_workload(fl, "powerlaw", m, n, dcp, dri, dv, x, variants, args.steps, args.warmup,
+                                             flush)
+        del mats
+        comparators = {} if args.no_compare else {
+            "powerlaw": run_comparators(fl, "powerlaw", m, n, dcp, dri, dv, x, y_ref, args.steps, args.warmup, flush)}
+        del dcp, dri, dv
 
-    ok = {k: r for k, r in res.items() if r["rel_err_l2"] <= 1e-4 and k.startswith("learned")}
-    best = max(ok, key=lambda k: ok[k]["gbs_equiv_f32"])
-    r = res[best]
-    plain = res.get("plain_f32")
-    kind_names = {"f32": "f32", "f64": "f64", "bf16": "bf16 values, f32 products, f64 accumulate",
-                  "bf16x2": "split hi+lo bf16 values, f32 products, f64 accumulate"}
-    out = {
-        "metric": "SpMV HBM GB/s (% of peak): learned+mixed-prec vs plain FastLoad at stated err",
-        "value": r["gbs_equiv_f32"],
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["step_ms"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": kind_names[best.split("_", 1)[1]],
-        "data": "synthetic",
-        "config": {"workload": f"power-law CSC SpMV {m}x{n}, Zipf({wl.POWERLAW_ZIPF}, max {wl.POWERLAW_MAX}) column "
-                               f"nnz, seed {args.seed}",
-                   "nnz": nnz, "variant": best, "learned_schedule": chosen, "rel_err_l2_vs_fp64": r["rel_err_l2"],
-                   "plain_f32_gbs": plain["gbs"] if plain else None,
-                   "speedup_vs_plain_f32": round(plain["step_ms"] / r["step_ms"], 3) if plain else None},
-        "roofline": {"bound": "hbm", "achieved": r["kernel_gbs_equiv_f32"], "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4),
-                     "traffic": profiled_traffic(chosen["schedule"], best.split("_", 1)[1], "powerlaw"),
-                     "flushed_achieved": r.get("kernel_gbs_equiv_f32_flushed"),
-                     "flushed_frac": (round(r["kernel_gbs_equiv_f32_flushed"] / PEAK_GBS, 4)
-                                      if "kernel_gbs_equiv_f32_flushed" in r else None)},
-        "variants": res,
-    }
-    if comparators:
-        lf = r["kernel_ms"]
-        for k, c in comparators["powerlaw"].items():
-            c["learned_speedup"] = round(c["kernel_ms"] / lf, 2)
+        ok = {k: r for k, r in res.items() if r["rel_err_l2"] <= 1e-4 and k.startswith("learned")}
+        best = max(ok, key=lambda k: ok[k]["gbs_equiv_f32"])
+        r = res[best]
+        plain = res.get("plain_f32")
+        kind_names = {"f32": "f32", "f64": "f64", "bf16": "bf16 values, f32 products, f64 accumulate",
+                      "bf16x2": "split hi+lo bf16 values, f32 products, f64 accumulate",
+                      "packed": "packed value+slot words (value rounded to the leftover mantissa bits), f32 products, "
+                                "f64 accumulate"}
+        out = {
+            "metric": "SpMV HBM GB/s (% of peak): learned+mixed-prec vs plain FastLoad at stated err",
+            "value": r["gbs_equiv_f32"],
+            "unit": "GB/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": r["step_ms"],
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": kind_names[best.split("_", 1)[1]],
+            "data": "synthetic",
+            "config": {"workload": f"power-law CSC SpMV {m}x{n}, Zipf({wl.POWERLAW_ZIPF}, max {wl.POWERLAW_MAX}) column "
+                                   f"nnz, seed {args.seed}",
+                       "nnz": nnz, "variant": best, "learned_schedule": chosen, "rel_err_l2_vs_fp64": r["rel_err_l2"],
+                       "plain_f32_gbs": plain["gbs"] if plain else None,
+                       "speedup_vs_plain_f32": round(plain["step_ms"] / r["step_ms"], 3) if plain else None},
+            "roofline": {"bound": "hbm", "achieved": r["kernel_gbs_equiv_f32"], "peak": PEAK_GBS, "unit": "GB/s",
+                         "frac": round(r["kernel_gbs_equiv_f32"] / PEAK_GBS, 4),
+                         "traffic": profiled_traffic(chosen["schedule"], best.split("_", 1)[1], "powerlaw"),
+                         "flushed_achieved": r.get("kernel_gbs_equiv_f32_flushed"),
+                         "flushed_frac": (round(r["kernel_gbs_equiv_f32_flushed"] / PEAK_GBS, 4)
+                                          if "kernel_gbs_equiv_f32_flushed" in r else None)},
+            "variants": res,
+        }
+        if comparators:
+            lf = r["kernel_ms"]
+            for k, c in comparators["powerlaw"].items():
+                c["learned_speedup"] = round(c["kernel_ms"] / lf, 2)
     if not args.no_stencil:
         sm, sn, scp, sri, sv, sx = wl.stencil(128)
         dcp, dri, dv = DeviceArray.from_host(scp), DeviceArray.from_host(sri), DeviceArray.from_host(sv)
@@ -672,9 +665,9 @@ def main():
         if not args.no_compare:
             comparators["stencil"] = run_comparators(fl, "stencil", sm, sn, dcp, dri, dv, sx, sy_ref, args.steps,
                                                      args.warmup, flush)
-            slf = sres["learned_f32"]["kernel_ms"]
+            slf = sres["learned_f32"]["kernel_ms"] if "learned_f32" in sres else None
             for c in comparators["stencil"].values():
-                c["learned_speedup"] = round(c["kernel_ms"] / slf, 2)
+                c["learned_speedup"] = round(c["kernel_ms"] / slf, 2) if slf else None
         out["stencil"] = {"workload": "27-point stencil on 128^3, |v| U[0.5,1.5] random sign, seed 128", "nnz": snnz,
                           "learned_schedule": schosen,
                           "traffic_bytes_learned_f32": profiled_traffic(schosen["schedule"], "f32", "stencil"),
@@ -686,12 +679,57 @@ def main():
     if not args.no_c1:
         out["c1"] = run_c1(fl, args.steps, args.warmup)
     if args.batch > 0:
-        out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup)
-    if not args.no_cpu:
+        out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup, sequential=not args.no_seq,
+                                     ks=tuple(int(k) for k in args.batch_k.split(",")))
+    if not args.no_cpu and not args.no_powerlaw:
         m, n, cp, ri, v, _ = wl.powerlaw(args.n, args.seed)
         M32 = fl.FastLoadMatrix(m, n, cp, ri, v, width=32)
         del cp, ri, v
         out["cpu_baseline"] = cpu_baseline(M32, m, n, nnz)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timings")
+    ap.add_argument("--no-compare", action="store_true", help="skip the naive-CSC and rocSPARSE yardsticks")
+    ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
+    ap.add_argument("--batch-k", default="1,8,32", help="right-hand-side counts of the batched configuration")
+    ap.add_argument("--no-seq", action="store_true", help="skip the batched configuration's sequential baselines")
+    ap.add_argument("--no-powerlaw", action="store_true", help="skip the power-law workload (profiling runs)")
+    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
+    ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,winr64_f32,pb_f32,row_f32")
+    ap.add_argument("--schedule", default=None, help="use this schedule for the learned variants instead of "
+                    "selecting one (profiling runs)")
+    ap.add_argument("--stencil-schedule", default=None, help="the same for the stencil workload")
+    ap.add_argument("--learned-kinds", default="f32,bf16x2,bf16,packed",
+                    help="value storages timed for the learned schedule")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this FastLoad build runs on one GPU only (no multi-GPU path is built); "
+                                   "run with --gpus 1"}))
+        sys.exit(2)
+
+    # The library's reference-style host API prints its check lines ("Check FASTLOAD GPU PASS!") on
+    # stdout; the one JSON line must be the only thing there, so fd 1 points at stderr while the
+    # workloads run.
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        out = run_all(args)
+    finally:
+        sys.stdout.flush()
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
     print(json.dumps(out), flush=True)
 
 

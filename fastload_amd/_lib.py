@@ -65,6 +65,7 @@ SIGNATURES = {
     "fl_permute_x_f64": (C.c_int, [vp, vp, vp, vp]),
     "fl_permute_x_f32": (C.c_int, [vp, vp, vp, vp]),
     "fl_matrix_prepare_values": (C.c_int, [vp, C.c_int, vp]),
+    "fl_matrix_pack_info": (C.c_int, [vp, C.c_int, i32p, i32p, i32p]),
     "fl_spmv": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
     "fl_spmv_assign": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
     "fl_matrix_prepare_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),

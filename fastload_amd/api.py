@@ -15,7 +15,7 @@ import numpy as np
 from ._lib import FastLoadError, MatrixInfo, SlideMatrix, check, i32p, f64p, lib
 from .device import DeviceArray
 
-VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2 = 0, 1, 2, 3
+VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2, VAL_PACKED = 0, 1, 2, 3, 4
 KERNEL_DROPIN, KERNEL_FAST, KERNEL_WIN, KERNEL_PB, KERNEL_WINR = 0, 1, 2, 3, 4
 
 GEN_SYMGEOM, GEN_ZIPF, GEN_STENCIL27, GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF_DENSE = range(7)
@@ -296,6 +296,12 @@ class FastLoadMatrix:
         check(lib.fl_matrix_prepare_values(self.handle, val_kind, stream.ptr if stream is not None else None),
               "fl_matrix_prepare_values")
 
+    def pack_info(self, kernel: int):
+        """(mantissa bits, exponent-code bits, slot bits) of the schedule's packed words."""
+        m, e, i = C.c_int(), C.c_int(), C.c_int()
+        check(lib.fl_matrix_pack_info(self.handle, kernel, C.byref(m), C.byref(e), C.byref(i)), "fl_matrix_pack_info")
+        return m.value, e.value, i.value
+
     def prepare_schedule(self, kernel: int, batch_tiles: int = 256, band_rows: int = 16384, stream=None):
         check(lib.fl_matrix_prepare_schedule(self.handle, kernel, batch_tiles, band_rows,
                                              stream.ptr if stream is not None else None), "fl_matrix_prepare_schedule")
@@ -325,7 +331,7 @@ class FastLoadMatrix:
     def spmv(self, dsx: DeviceArray, dy: DeviceArray, kernel=KERNEL_DROPIN, val_kind=VAL_F64, work=None, stream=None,
              assign=False):
         """y += A * sortx on the device (y = A * sortx with assign=True)."""
-        if val_kind not in (VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2):
+        if val_kind not in (VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2, VAL_PACKED):
             raise FastLoadError(-8, f"spmv: unknown val_kind {val_kind}", from_library=False)
         _need(dsx, self.n, np.float64 if val_kind == VAL_F64 else np.float32, "spmv sortx")
         _need(dy, self.m, np.float64 if val_kind in (VAL_F64, VAL_BF16X2) else np.float32, "spmv y")

@@ -540,10 +540,24 @@ int fl_permute_x_f32(const fl_matrix *A, const float *d_x, float *d_sortx, void 
 int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream) {
     FL_API_BEGIN
     if (!A) return invalid("fl_matrix_prepare_values: NULL");
-    if (val_kind < FL_VAL_F64 || val_kind > FL_VAL_BF16X2) return invalid("fl_matrix_prepare_values: bad kind");
+    if (val_kind < FL_VAL_F64 || val_kind > FL_VAL_PACKED) return invalid("fl_matrix_prepare_values: bad kind");
     prepare_values(A, val_kind, (hipStream_t)stream);
     return FL_OK;
     FL_API_END
+}
+
+int fl_matrix_pack_info(const fl_matrix *A, int kernel, int *mantissa_bits, int *exponent_bits, int *index_bits) {
+    if (!A) return invalid("fl_matrix_pack_info: NULL");
+    const bool pb = kernel == FL_KERNEL_PB;
+    if (!pb && kernel != FL_KERNEL_WIN && kernel != FL_KERNEL_WINR)
+        return invalid("fl_matrix_pack_info: kernel must be WIN, WINR or PB");
+    if (!(pb ? A->pb_pack.p : A->win_pack.p) || (!pb && A->ord_kind != (kernel == FL_KERNEL_WINR ? 1 : 0)))
+        return invalid("fl_matrix_pack_info: no packed values for this schedule");
+    const PackFmt &f = pb ? A->pb_pf : A->win_pf;
+    if (mantissa_bits) *mantissa_bits = f.m;
+    if (exponent_bits) *exponent_bits = f.e;
+    if (index_bits) *index_bits = f.ib;
+    return FL_OK;
 }
 
 int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int band_rows, void *stream) {

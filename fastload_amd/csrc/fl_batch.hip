@@ -66,6 +66,27 @@ __global__ void k_row_bounds(const uint32_t *__restrict__ k, int count, int m, i
     }
 }
 
+// out[k] = first row whose entries start at or after k/8 of nnz (out[0] = 0, out[8] = m)
+__global__ void k_row_split(const int *__restrict__ row_ptr, int m, int nnz, int *__restrict__ out) {
+    const int k = threadIdx.x;
+    if (k > 8) return;
+    const long long target = (long long)nnz * k / 8;
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (row_ptr[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    out[k] = k == 8 ? m : lo;
+}
+
+static void compute_row_split(fl_matrix *A, hipStream_t s) {
+    DevBuf<int> d(9);
+    k_row_split<<<1, 64, 0, s>>>(A->row_ptr.p, A->m, A->nnz, d.p);
+    FL_LAUNCH_CHECK();
+    FL_HIP(hipMemcpyAsync(A->row_split.r, d.p, sizeof(int) * 9, hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+}
+
 void row_prepare(fl_matrix *A, hipStream_t s) {
     if (A->row_ready) return;
     const int nnz = A->nnz, m = A->m, T = A->tilenum;
@@ -75,7 +96,7 @@ void row_prepare(fl_matrix *A, hipStream_t s) {
     A->row_val64.alloc(std::max(nnz, 1));
     if (nnz == 0) {
         FL_HIP(hipMemsetAsync(A->row_ptr.p, 0, sizeof(int) * ((size_t)m + 1), s));
-        FL_HIP(hipStreamSynchronize(s));
+        compute_row_split(A, s);
         A->row_ready = true;
         return;
     }
@@ -96,7 +117,7 @@ void row_prepare(fl_matrix *A, hipStream_t s) {
                                             A->row_val32.p, nnz);
     k_row_bounds<<<sgb((long long)m + 1), kBlock, 0, s>>>(key.p, nnz, m, A->row_ptr.p);
     FL_LAUNCH_CHECK();
-    FL_HIP(hipStreamSynchronize(s));
+    compute_row_split(A, s);
     A->row_ready = true;
 }
 
@@ -126,7 +147,7 @@ int row_install(fl_matrix *A, const int *row_ptr, const int *row_col, const doub
         k_row_val32<<<sgb(nnz), kBlock, 0, s>>>(A->row_val64.p, A->row_val32.p, nnz);
         FL_LAUNCH_CHECK();
     }
-    FL_HIP(hipStreamSynchronize(s));
+    compute_row_split(A, s);
     A->row_ready = true;
     return FL_OK;
 }
@@ -144,8 +165,27 @@ template <> struct RowVec<double, 2> { using type = double2; };
 // [j*VEC, j*VEC + VEC) of the row's Y block, gathering them from X as one VEC-wide load per
 // entry. Every Y element sums its row's entries in entry order whatever VEC is. Persistent:
 // waves stride over row groups. ACC = accumulator type.
+// The waves of XCD x (workgroups with blockIdx % 8 == x; gridDim a multiple of 8) stride over the
+// row groups of that XCD's range [sp.r[x], sp.r[x+1]), so the rows in flight on one XCD at any
+// time are a contiguous window, and on matrices whose rows have local columns (bands, stencils,
+// block-diagonal batches) the X rows they gather stay in that XCD's L2.
+struct XcdRows {
+    long long wave, nwaves;
+    int r0, r1;
+};
+
+__device__ __forceinline__ XcdRows xcd_rows(const RowSplit &sp) {
+    XcdRows w;
+    const int xcd = blockIdx.x & 7;
+    w.wave = uni((int)((((blockIdx.x >> 3) * blockDim.x) + threadIdx.x) >> 6));  // wave-uniform loop
+    w.nwaves = ((long long)(gridDim.x >> 3) * blockDim.x) >> 6;
+    w.r0 = sp.r[xcd];
+    w.r1 = sp.r[xcd + 1];
+    return w;
+}
+
 template <typename T, typename ACC, int K, int VEC, int UNR>
-__global__ __launch_bounds__(kBlock) void k_spmm_row(int m, const int *__restrict__ row_ptr,
+__global__ __launch_bounds__(kBlock) void k_spmm_row(RowSplit sp, const int *__restrict__ row_ptr,
                                                      const int *__restrict__ row_col, const T *__restrict__ row_val,
                                                      const T *__restrict__ X, T *__restrict__ Y, int accumulate) {
     using V = typename RowVec<T, VEC>::type;
@@ -154,12 +194,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_row(int m, const int *__restric
     const int lane = threadIdx.x & 63;
     const int sub = lane / L, j = lane % L;
     const V *__restrict__ Xv = reinterpret_cast<const V *>(X);
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    const long long groups = ((long long)m + RPW - 1) / RPW;
-    for (long long gq = wave; gq < groups; gq += nwaves) {
-        const long long r = gq * RPW + sub;
-        if (r >= m) continue;
+    const XcdRows xw = xcd_rows(sp);
+    const long long groups = ((long long)(xw.r1 - xw.r0) + RPW - 1) / RPW;
+    for (long long gq = xw.wave; gq < groups; gq += xw.nwaves) {
+        const long long r = xw.r0 + gq * RPW + sub;
+        if (r >= xw.r1) continue;
         const int e0 = row_ptr[r], e1 = row_ptr[r + 1];
         ACC acc[VEC];
 #pragma unroll
@@ -198,19 +237,19 @@ __global__ __launch_bounds__(kBlock) void k_spmm_row(int m, const int *__restric
 // of 0.88 ms with one lane per row (0.67 ms with 2 lanes, 0.59 ms with 8, which loses on short
 // rows: 9.3 vs 12.2 us on a 2.4 M-entry block-diagonal matrix of 6-entry rows).
 template <typename T, typename ACC, int LPR, int UNR>
-__global__ __launch_bounds__(kBlock) void k_spmv_rowl(int m, const int *__restrict__ row_ptr,
+__global__ __launch_bounds__(kBlock) void k_spmv_rowl(RowSplit sp, const int *__restrict__ row_ptr,
                                                       const int *__restrict__ row_col, const T *__restrict__ row_val,
                                                       const T *__restrict__ X, T *__restrict__ Y, int accumulate) {
     constexpr int RPW = 64 / LPR;
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR, j = lane % LPR;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    const long long groups = ((long long)m + RPW - 1) / RPW;
-    for (long long gq = wave; gq < groups; gq += nwaves) {  // uniform per wave: all lanes reach the butterfly
-        const long long r = gq * RPW + sub;
+    const XcdRows xw = xcd_rows(sp);
+    const long long groups = ((long long)(xw.r1 - xw.r0) + RPW - 1) / RPW;
+    for (long long gq = xw.wave; gq < groups; gq += xw.nwaves) {  // uniform per wave: all lanes reach the butterfly
+        const long long r = xw.r0 + gq * RPW + sub;
+        const bool live = r < xw.r1;
         int e0 = 0, e1 = 0;
-        if (r < m) {
+        if (live) {
             e0 = row_ptr[r];
             e1 = row_ptr[r + 1];
         }
@@ -232,25 +271,31 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rowl(int m, const int *__restri
         for (; e < e1; e += LPR) acc += (ACC)row_val[e] * (ACC)X[row_col[e]];
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-        if (r < m && j == 0) Y[r] = accumulate ? (T)((ACC)Y[r] + acc) : (T)acc;
+        if (live && j == 0) Y[r] = accumulate ? (T)((ACC)Y[r] + acc) : (T)acc;
     }
+}
+
+// Persistent grid: enough waves for the rows, at most 2048 workgroups, a multiple of 8 (one share
+// per XCD).
+static unsigned row_grid(long long m, int rows_per_wave) {
+    const long long groups = (m + rows_per_wave - 1) / rows_per_wave;
+    const unsigned g = std::min<unsigned>(grid_for(groups * 64), 256u * 8u);
+    return std::max(8u, (g + 7u) & ~7u);
 }
 
 template <typename T, typename ACC>
 static void launch_row_k1(const fl_matrix *A, const T *val, const T *X, T *Y, int accumulate, hipStream_t s) {
     constexpr int LPR = 4;
-    const long long groups = ((long long)A->m + (64 / LPR) - 1) / (64 / LPR);
-    const unsigned g = std::max(1u, std::min<unsigned>(grid_for(groups * 64), 256u * 8u));
-    k_spmv_rowl<T, ACC, LPR, 4><<<g, kBlock, 0, s>>>(A->m, A->row_ptr.p, A->row_col.p, val, X, Y, accumulate);
+    k_spmv_rowl<T, ACC, LPR, 4><<<row_grid(A->m, 64 / LPR), kBlock, 0, s>>>(A->row_split, A->row_ptr.p, A->row_col.p,
+                                                                            val, X, Y, accumulate);
     FL_LAUNCH_CHECK();
 }
 
 template <typename T, typename ACC, int K, int VEC>
 static void launch_row_v(const fl_matrix *A, const T *val, const T *X, T *Y, int accumulate, hipStream_t s) {
     constexpr int RPW = 64 * VEC / K;
-    const long long groups = ((long long)A->m + RPW - 1) / RPW;
-    const unsigned g = std::max(1u, std::min<unsigned>(grid_for(groups * 64), 256u * 8u));
-    k_spmm_row<T, ACC, K, VEC, 4><<<g, kBlock, 0, s>>>(A->m, A->row_ptr.p, A->row_col.p, val, X, Y, accumulate);
+    k_spmm_row<T, ACC, K, VEC, 4><<<row_grid(A->m, RPW), kBlock, 0, s>>>(A->row_split, A->row_ptr.p, A->row_col.p, val,
+                                                                         X, Y, accumulate);
     FL_LAUNCH_CHECK();
 }
 

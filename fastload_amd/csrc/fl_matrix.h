@@ -6,6 +6,29 @@
 #include "fl_common.h"
 #include "fl_walk.h"
 
+// Packed entries of the batched schedules (FL_VAL_PACKED): one 32-bit word per entry holding the
+// value rounded to `m` mantissa bits and the schedule's LDS index (WIN slot / PB staging slot):
+//   bit 31 sign | [30 .. ib+m] exponent code (e bits) | [ib+m-1 .. ib] mantissa | [ib-1 .. 0] index
+// exponent code 0 = zero; code c > 0 = the FP32 biased exponent c + ebase.
+struct PackFmt {
+    uint32_t imask = 0, fmask = 0, mmask = 0, ebias = 0;
+    int ib = 0, e = 0, m = 0, ebase = 0;
+};
+
+// f = exponent code and mantissa; FP32 bits = (f << (23 - m)) + (ebase << 23) with the sign bit
+// copied; code 0 (f <= mmask) is zero. Five VALU operations.
+__device__ __forceinline__ float pack_decode(uint32_t w, const PackFmt &p) {
+    const uint32_t f = (w >> p.ib) & p.fmask;
+    const uint32_t bits = ((f << (23 - p.m)) + p.ebias) | (w & 0x80000000u);
+    return f > p.mmask ? __uint_as_float(bits) : 0.0f;
+}
+
+// Contiguous row ranges of the row-ordered kernels, one per XCD, of about equal entry counts:
+// XCD x (the workgroups with blockIdx % 8 == x) owns rows [r[x], r[x+1]).
+struct RowSplit {
+    int r[9];
+};
+
 struct fl_matrix {
     int m = 0, n = 0, nnz = 0, W = 32;
     int L = 0;        // levels (h_count)
@@ -63,9 +86,19 @@ struct fl_matrix {
     fl::DevBuf<float> pb_bins32;
     fl::DevBuf<double> pb_bins64;
 
+    // FL_VAL_PACKED: value + index words of the prepared WIN and PB schedules (built by
+    // fl_matrix_prepare_values(FL_VAL_PACKED) and for schedules prepared after it), their formats,
+    // and the matrix's exponent range (emin/emax of the non-zero values' FP32 exponents; pack_ok
+    // false when a value is not finite or too large for FP32)
+    bool pack_wanted = false, pack_range_known = false, pack_ok = false;
+    int pack_emin = 0, pack_emax = 0;
+    fl::DevBuf<uint32_t> win_pack, pb_pack;
+    PackFmt win_pf, pb_pf;
+
     // ROW (multi-right-hand-side): the entries re-ordered by row (CSR of A, columns in the
     // original numbering, so X needs no permutation): row_ptr [m+1], row_col/row_val [nnz].
     bool row_ready = false;
+    RowSplit row_split{};
     fl::DevBuf<int> row_ptr, row_col;
     fl::DevBuf<float> row_val32;
     fl::DevBuf<double> row_val64;

@@ -20,6 +20,7 @@
 //   each band one workgroup that sums its products per row in LDS (ds_add_f64) and updates y
 //   once per row with a coalesced read-modify-write -- no global atomics at all.
 #include <algorithm>
+#include <type_traits>
 
 #include "fl_matrix.h"
 
@@ -303,6 +304,8 @@ static void ensure_batches(fl_matrix *A, int bt, int kind, hipStream_t s) {
     A->bt = bt;
     A->nbatches = (T + bt - 1) / bt;
     A->win_ready = A->pb_ready = false;
+    A->win_pack.release();
+    A->pb_pack.release();
     A->batch_of.alloc(std::max(A->nnz, 1));
     A->batch_start.alloc((size_t)A->nbatches + 1);
     DevBuf<int> sizes(std::max(A->nbatches, 1));
@@ -315,9 +318,12 @@ static void ensure_batches(fl_matrix *A, int bt, int kind, hipStream_t s) {
     FL_HIP(hipStreamSynchronize(s));
 }
 
+void pack_schedules(fl_matrix *A, hipStream_t s);
+
 void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
     ensure_batches(A, bt, rows ? 1 : 0, s);
     if (A->win_ready) return;
+    A->win_pack.release();
     const int nnz = A->nnz, nb = A->nbatches;
     A->win_slot.alloc(std::max(nnz, 1));
     A->win_off.alloc((size_t)nb + 1);
@@ -371,6 +377,7 @@ void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
     FL_LAUNCH_CHECK();
     A->win_maxslots = device_max_i32(per_batch.p, nb, s);
     A->win_ready = true;
+    if (A->pack_wanted) pack_schedules(A, s);
 }
 
 // Chunk boundaries: C consecutive batch ranges of about nnz/C entries each, cut at multiples of
@@ -397,6 +404,8 @@ static std::vector<int> pb_chunk_batches(const fl_matrix *A, int C, hipStream_t 
 void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     ensure_batches(A, bt, 0, s);
     if (A->pb_ready && A->pb_R == R) return;
+    A->pb_ready = false;
+    A->pb_pack.release();
     const int nnz = A->nnz, nb = A->nbatches, m = A->m;
     const int nbands = std::max((m + R - 1) / R, 1);
     if ((long long)nb * nbands > 0x7fffffffLL) {
@@ -496,6 +505,110 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     A->pb_chunk_batch = cb;
     A->pb_chunk_off = off;
     A->pb_ready = true;
+    if (A->pack_wanted) pack_schedules(A, s);
+}
+
+// ------------------------------------------------------------- packed value + slot words
+// Exponent range of the non-zero values: r[0] = min, r[1] = max unbiased exponent among values
+// of FP32-normal magnitude (smaller ones are stored as zero), r[2] != 0 when a value is not
+// finite or too large to pack (exponent >= 127).
+__global__ void k_exp_range(const double *__restrict__ val, int count, int *__restrict__ r) {
+    int lo = 0x7fffffff, hi = -0x7fffffff, bad = 0;
+    GRID_STRIDE(g, count) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(val[g]);
+        const int ed = (int)((b >> 52) & 0x7ff) - 1023;
+        if (ed >= 127) bad = 1;
+        else if (ed >= -126) {
+            lo = min(lo, ed);
+            hi = max(hi, ed);
+        }
+    }
+    atomicMin(&r[0], lo);
+    atomicMax(&r[1], hi);
+    if (bad) atomicOr(&r[2], 1);
+}
+
+// word = sign | exponent code | mantissa rounded to nearest even (from the FP64 value) | index
+__global__ void k_pack(const double *__restrict__ val, const uint16_t *__restrict__ idx, int count, PackFmt f,
+                       int emin, uint32_t *__restrict__ out) {
+    GRID_STRIDE(g, count) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(val[g]);
+        const int ed = (int)((b >> 52) & 0x7ff) - 1023;
+        uint32_t word = 0;
+        if (ed >= -126) {
+            const int sh = 52 - f.m;
+            const unsigned long long frac = b & ((1ull << 52) - 1ull);
+            unsigned long long q = frac >> sh;
+            const unsigned long long rem = frac & ((1ull << sh) - 1ull), half = 1ull << (sh - 1);
+            int e = ed;
+            if (rem > half || (rem == half && (q & 1ull))) ++q;
+            if (q >> f.m) {  // rounding carried into the exponent
+                q = 0;
+                ++e;
+            }
+            const uint32_t sign = (uint32_t)(b >> 63), ec = (uint32_t)(e - emin + 1);
+            word = (sign << (f.ib + f.m + f.e)) | (ec << (f.ib + f.m)) | ((uint32_t)q << f.ib);
+        }
+        out[g] = word | (uint32_t)idx[g];
+    }
+}
+
+// sign at bit 31, then e exponent-code bits, m mantissa bits and ib slot bits (1 + e + m + ib = 32):
+// e covers the exponent range (one code more for a rounding carry; code 0 = zero), m takes what
+// is left up to 23, and e absorbs any bits beyond that.
+static bool pack_format(const fl_matrix *A, int ib, PackFmt &f) {
+    const int e_need = bits_for((uint32_t)(A->pack_emax - A->pack_emin + 2));
+    const int m = std::min(23, 31 - ib - e_need);
+    if (!A->pack_ok || m < 12) return false;
+    f.ib = ib;
+    f.m = m;
+    f.e = 31 - ib - m;
+    f.ebase = A->pack_emin + 126;
+    f.imask = (1u << ib) - 1u;
+    f.fmask = (f.e + m >= 32) ? ~0u : (1u << (f.e + m)) - 1u;
+    f.mmask = (1u << m) - 1u;
+    f.ebias = (uint32_t)f.ebase << 23;
+    return true;
+}
+
+// Builds the packed words of every prepared batched schedule that has none yet. A schedule whose
+// slot bits leave fewer than 12 mantissa bits (or a matrix with non-finite or huge values) keeps
+// no packed words, and FL_VAL_PACKED calls on it are refused.
+void pack_schedules(fl_matrix *A, hipStream_t s) {
+    const int nnz = A->nnz;
+    if (!A->pack_range_known) {
+        DevBuf<int> r(3);
+        const int init[3] = {0x7fffffff, -0x7fffffff, 0};
+        FL_HIP(hipMemcpyAsync(r.p, init, sizeof init, hipMemcpyHostToDevice, s));
+        if (nnz) k_exp_range<<<sg(nnz), kBlock, 0, s>>>(A->val64.p, nnz, r.p);
+        FL_LAUNCH_CHECK();
+        int h[3];
+        FL_HIP(hipMemcpyAsync(h, r.p, sizeof h, hipMemcpyDeviceToHost, s));
+        FL_HIP(hipStreamSynchronize(s));
+        A->pack_ok = h[2] == 0;
+        A->pack_emin = h[0] <= h[1] ? h[0] : 0;
+        A->pack_emax = h[0] <= h[1] ? h[1] : 0;
+        A->pack_range_known = true;
+    }
+    if (A->win_ready && !A->win_pack.p) {
+        PackFmt f;
+        if (pack_format(A, std::max(1, bits_for((uint32_t)std::max(A->win_maxslots - 1, 0))), f)) {
+            A->win_pack.alloc(std::max(nnz, 1));
+            if (nnz) k_pack<<<sg(nnz), kBlock, 0, s>>>(A->val64.p, A->win_slot.p, nnz, f, A->pack_emin, A->win_pack.p);
+            FL_LAUNCH_CHECK();
+            A->win_pf = f;
+        }
+    }
+    if (A->pb_ready && !A->pb_pack.p) {
+        PackFmt f;
+        if (pack_format(A, bits_for((uint32_t)(A->bt * 64 - 1)), f)) {
+            A->pb_pack.alloc(std::max(nnz, 1));
+            if (nnz) k_pack<<<sg(nnz), kBlock, 0, s>>>(A->val64.p, A->pb_lslot.p, nnz, f, A->pack_emin, A->pb_pack.p);
+            FL_LAUNCH_CHECK();
+            A->pb_pf = f;
+        }
+    }
+    FL_HIP(hipStreamSynchronize(s));
 }
 
 // ----------------------------------------------------------- schedules read back from a file
@@ -547,6 +660,7 @@ int win_install(fl_matrix *A, int bt, bool rows, int nb_file, const uint16_t *sl
     }
     A->win_maxslots = maxslots;
     A->win_ready = true;
+    A->win_pack.release();
     return FL_OK;
 }
 
@@ -602,6 +716,7 @@ int pb_install(fl_matrix *A, int bt, int R, int nb_file, int C, const int *chunk
     A->pb_chunk_batch.assign(chunk_batch, chunk_batch + C + 1);
     A->pb_chunk_off.assign(chunk_off, chunk_off + C);
     A->pb_ready = true;
+    A->pb_pack.release();
     return FL_OK;
 }
 
@@ -627,7 +742,7 @@ struct WaveTiles {
 
 template <int NT>
 __device__ __forceinline__ WaveTiles wave_tiles(const OrdTiles &ot, int bt, int batch) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = uni((int)(threadIdx.x >> 6));  // wave-uniform: scalar tile bounds
     const int tpw = bt / (NT / 64);
     WaveTiles w;
     w.t0 = batch * bt + wave * tpw;
@@ -638,11 +753,16 @@ __device__ __forceinline__ WaveTiles wave_tiles(const OrdTiles &ot, int bt, int 
     return w;
 }
 
-// Raw operands of U tiles of one wave (value or hi+lo BF16 word, x, LDS slot; slot -1 = no entry),
-// loaded one group ahead of the group being accumulated.
-template <typename V, typename X, bool SPLIT, int U>
+// Value modes of the batched kernels: kPlain -- a value array (double, float or BF16) next to the
+// 16-bit slot array; kSplit -- hi+lo BF16 pair in one 32-bit word (hi << 16 | lo) next to the slot
+// array; kPacked -- value and slot in one 32-bit word (PackFmt), no slot array.
+constexpr int kPlain = 0, kSplit = 1, kPacked = 2;
+
+// Raw operands of U tiles of one wave (value, x, LDS slot; slot -1 = no entry), loaded one group
+// ahead of the group being accumulated. Packed words are decoded to float as they are loaded.
+template <typename V, typename X, int MODE, int U>
 struct TileGroup {
-    V v[U];
+    std::conditional_t<MODE == kPacked, float, V> v[U];
     X xv[U];
     int sl[U];
 };
@@ -653,11 +773,21 @@ __device__ __forceinline__ auto widen(V v) {
     else return v;
 }
 
-template <typename V, typename X, bool SPLIT, int U>
+// base[l] for a wave-uniform base and a small lane index, addressed as base + 32-bit byte offset.
+template <typename T>
+__device__ __forceinline__ T lane_load(const T *__restrict__ base, uint32_t l) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + l * (uint32_t)sizeof(T));
+}
+
+// Tile u of the group: its entries are lanes [0, len) at entry offset ptr and sorted column cx (all
+// uniform). Lanes past the tile read its first entry instead (every array has one when the matrix
+// has tiles) and are given the slot `dummy + lane`, a per-lane spare LDS element past the batch's
+// slots, so that loads, products and LDS updates need no branches and no two spare lanes collide.
+template <typename V, typename X, int MODE, int U>
 __device__ __forceinline__ void load_group(const OrdTiles &ot, const WaveTiles &w, int t,
                                            const uint16_t *__restrict__ slot, const V *__restrict__ val,
-                                           const X *__restrict__ x,
-                                           TileGroup<V, X, SPLIT, U> &G) {
+                                           const X *__restrict__ x, const PackFmt &pf, int dummy,
+                                           TileGroup<V, X, MODE, U> &G) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -668,20 +798,28 @@ __device__ __forceinline__ void load_group(const OrdTiles &ot, const WaveTiles &
             len = min(64, ot.n - cx);
         }
         const bool act = lane < len;
-        const int g = ptr + lane;
-        G.sl[u] = act ? (int)slot[g] : -1;
-        G.v[u] = act ? val[g] : (V)0;
-        G.xv[u] = act ? x[cx + lane] : (X)0;
+        const uint32_t l = act ? (uint32_t)lane : 0u;
+        // uniform base (scalar registers) + 32-bit lane byte offset: no 64-bit address arithmetic
+        if constexpr (MODE == kPacked) {
+            const uint32_t wd = lane_load(val + ptr, l);
+            G.sl[u] = act ? (int)(wd & pf.imask) : dummy + lane;
+            G.v[u] = pack_decode(wd, pf);
+        } else {
+            const uint16_t sv = lane_load(slot + ptr, l);
+            G.sl[u] = act ? (int)sv : dummy + lane;
+            G.v[u] = lane_load(val + ptr, l);
+        }
+        G.xv[u] = lane_load(x + cx, l);
     }
 }
 
 // Product of entry u in FP64 (FP64 mode) or as the FP64 sum of FP32 products (FP32 / BF16 /
 // hi+lo BF16).
-template <typename V, typename X, bool SPLIT, int U>
-__device__ __forceinline__ double group_product(const TileGroup<V, X, SPLIT, U> &G, int u) {
+template <typename V, typename X, int MODE, int U>
+__device__ __forceinline__ double group_product(const TileGroup<V, X, MODE, U> &G, int u) {
     if constexpr (sizeof(X) == 8) {
         return (double)widen(G.v[u]) * G.xv[u];
-    } else if constexpr (SPLIT) {  // hi+lo BF16 pair in one word: hi << 16 | lo
+    } else if constexpr (MODE == kSplit) {  // hi+lo BF16 pair in one word: hi << 16 | lo
         const float hi = __uint_as_float(G.v[u] & 0xffff0000u), lo = __uint_as_float(G.v[u] << 16);
         return (double)(hi * G.xv[u]) + (double)(lo * G.xv[u]);
     } else {
@@ -689,20 +827,18 @@ __device__ __forceinline__ double group_product(const TileGroup<V, X, SPLIT, U> 
     }
 }
 
-// WIN: add each product to its row's LDS slot.
-template <typename V, typename X, bool SPLIT, int U>
-__device__ __forceinline__ void add_group(double *acc, const TileGroup<V, X, SPLIT, U> &G) {
+// WIN: add each product to its row's LDS slot (lanes past a tile add to their spare slot).
+template <typename V, typename X, int MODE, int U>
+__device__ __forceinline__ void add_group(double *acc, const TileGroup<V, X, MODE, U> &G) {
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (G.sl[u] >= 0) atomicAdd(&acc[G.sl[u]], group_product(G, u));
+    for (int u = 0; u < U; ++u) atomicAdd(&acc[G.sl[u]], group_product(G, u));
 }
 
-// PB phase 1: store each product at its band-major staging slot.
-template <typename V, typename X, bool SPLIT, int U, typename P>
-__device__ __forceinline__ void stage_group(P *stage, const TileGroup<V, X, SPLIT, U> &G) {
+// PB phase 1: store each product at its band-major staging slot (or its lane's spare slot).
+template <typename V, typename X, int MODE, int U, typename P>
+__device__ __forceinline__ void stage_group(P *stage, const TileGroup<V, X, MODE, U> &G) {
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (G.sl[u] >= 0) stage[G.sl[u]] = (P)group_product(G, u);
+    for (int u = 0; u < U; ++u) stage[G.sl[u]] = (P)group_product(G, u);
 }
 
 // One batch per workgroup: LDS slots of the batch's distinct rows are cleared, every entry adds
@@ -711,9 +847,9 @@ __device__ __forceinline__ void stage_group(P *stage, const TileGroup<V, X, SPLI
 // the first group before the barrier that ends the slot clearing.
 // ASSIGN: y was cleared before the launch and this is its only accumulation, so a row owned by
 // this batch is stored without reading y first (shared rows still take an atomic).
-template <typename V, typename X, typename Y, bool SPLIT, int U, int BT, int NT, bool ASSIGN>
+template <typename V, typename X, typename Y, int MODE, int U, int BT, int NT, bool ASSIGN>
 __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t *__restrict__ slot,
-                                            const V *__restrict__ val,
+                                            const V *__restrict__ val, PackFmt pf, int spare,
                                             const X *__restrict__ x, const int *__restrict__ win_rows,
                                             const int *__restrict__ win_off, Y *__restrict__ y) {
     extern __shared__ double acc[];
@@ -730,14 +866,14 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
         const int i = threadIdx.x + k * NT;
         frow[k] = (i < ns) ? win_rows[s0 + i] : 0;
     }
-    TileGroup<V, X, SPLIT, U> cur, nxt;
-    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, slot, val, x, cur);
+    TileGroup<V, X, MODE, U> cur, nxt;
+    if (w.t0 < w.t1) load_group<V, X, MODE, U>(ot, w, w.t0, slot, val, x, pf, spare, cur);
     for (int i = threadIdx.x; i < ns; i += NT) acc[i] = 0.0;
     __syncthreads();
     for (int t = w.t0; t < w.t1; t += U) {
         const bool more = t + U < w.t1;
-        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, slot, val, x, nxt);
-        add_group<V, X, SPLIT, U>(acc, cur);
+        if (more) load_group<V, X, MODE, U>(ot, w, t + U, slot, val, x, pf, spare, nxt);
+        add_group<V, X, MODE, U>(acc, cur);
         if (more) cur = nxt;
     }
     __syncthreads();
@@ -760,16 +896,16 @@ __global__ __launch_bounds__(NT) void k_win(OrdTiles ot, int bt, const uint16_t 
 
 // Phase 1 over batches [boff, boff + gridDim.x) (one chunk); boff is a multiple of 8, so batch b
 // runs on XCD b % 8 as the run order of the product array assumes.
-template <typename V, typename X, typename P, bool SPLIT, int U, int NT = kBigBlock>
+template <typename V, typename X, typename P, int MODE, int U, int NT = kBigBlock>
 __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const uint16_t *__restrict__ lslot,
-                                                   const V *__restrict__ val,
+                                                   const V *__restrict__ val, PackFmt pf,
                                                    const X *__restrict__ x, const int *__restrict__ run_off,
                                                    const int *__restrict__ run_lstart, const int *__restrict__ run_dest,
                                                    const int *__restrict__ run_len, P *__restrict__ bins) {
     extern __shared__ double stage_raw[];
     P *stage = reinterpret_cast<P *>(stage_raw);
     const int b = boff + blockIdx.x;
-    const int wave = threadIdx.x >> 6;
+    const int wave = uni((int)(threadIdx.x >> 6));
     // Wave w copies runs r0 + w + (NT/64)*j. Prefetch the metadata of its first 64 runs (lane j holds
     // run j) before the tile loop so the copy loop only reads registers.
     const int r0 = run_off[b], r1 = run_off[b + 1];
@@ -784,12 +920,12 @@ __global__ __launch_bounds__(NT) void k_pb1(OrdTiles ot, int bt, int boff, const
     }
     // tile loop, operands loaded one group ahead of the group being staged
     const WaveTiles w = wave_tiles<NT>(ot, bt, b);
-    TileGroup<V, X, SPLIT, U> cur, nxt;
-    if (w.t0 < w.t1) load_group<V, X, SPLIT, U>(ot, w, w.t0, lslot, val, x, cur);
+    TileGroup<V, X, MODE, U> cur, nxt;
+    if (w.t0 < w.t1) load_group<V, X, MODE, U>(ot, w, w.t0, lslot, val, x, pf, bt * 64, cur);
     for (int t = w.t0; t < w.t1; t += U) {
         const bool more = t + U < w.t1;
-        if (more) load_group<V, X, SPLIT, U>(ot, w, t + U, lslot, val, x, nxt);
-        stage_group<V, X, SPLIT, U>(stage, cur);
+        if (more) load_group<V, X, MODE, U>(ot, w, t + U, lslot, val, x, pf, bt * 64, nxt);
+        stage_group<V, X, MODE, U>(stage, cur);
         if (more) cur = nxt;
     }
     __syncthreads();
@@ -919,39 +1055,45 @@ static void allow_lds(K kern, size_t bytes) {
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-template <typename V, typename X, typename Y, bool SPLIT, int NT, bool ASSIGN>
+template <typename V, typename X, typename Y, int MODE, int NT, bool ASSIGN>
 static auto win_kernel(int bt) {
     constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group: 8 keeps the 4-byte-x kinds within 64 VGPRs
-    return (bt == 256) ? k_win<V, X, Y, SPLIT, U, 256, NT, ASSIGN>
-           : (bt == 128) ? k_win<V, X, Y, SPLIT, U, 128, NT, ASSIGN> : k_win<V, X, Y, SPLIT, U, 64, NT, ASSIGN>;
+    return (bt == 256) ? k_win<V, X, Y, MODE, U, 256, NT, ASSIGN>
+           : (bt == 128) ? k_win<V, X, Y, MODE, U, 128, NT, ASSIGN> : k_win<V, X, Y, MODE, U, 64, NT, ASSIGN>;
 }
 
-template <typename V, typename X, typename Y, bool SPLIT>
+template <typename V, typename X, typename Y, int MODE>
 static void launch_win(const fl_matrix *A, const V *val, const X *x, Y *y, bool assign,
                        hipStream_t s) {
+    const PackFmt pf = A->win_pf;
+    const uint16_t *slot = MODE == kPacked ? nullptr : A->win_slot.p;
     // LDS for the most distinct rows any batch has (at most bt*64). Workgroups of 4 waves per 64
     // tiles (16 tiles per wave; 512 threads at most): more, smaller workgroups per CU overlap
     // the three phases of a batch (slot set-up, accumulation, flush) better than 1024 threads.
-    const size_t lds = (size_t)std::max(A->win_maxslots, 1) * sizeof(double);
+    // + 64 spare slots, one per lane, for the lanes past a tile's end
+    const size_t lds = ((size_t)std::max(A->win_maxslots, 1) + 64) * sizeof(double);
     const int nt = std::min(A->bt * 4, 512);
-    auto kern = (nt == 256) ? (assign ? win_kernel<V, X, Y, SPLIT, 256, true>(A->bt)
-                                      : win_kernel<V, X, Y, SPLIT, 256, false>(A->bt))
-                            : (assign ? win_kernel<V, X, Y, SPLIT, 512, true>(A->bt)
-                                      : win_kernel<V, X, Y, SPLIT, 512, false>(A->bt));
+    auto kern = (nt == 256) ? (assign ? win_kernel<V, X, Y, MODE, 256, true>(A->bt)
+                                      : win_kernel<V, X, Y, MODE, 256, false>(A->bt))
+                            : (assign ? win_kernel<V, X, Y, MODE, 512, true>(A->bt)
+                                      : win_kernel<V, X, Y, MODE, 512, false>(A->bt));
     allow_lds(kern, lds);
-    kern<<<A->nbatches, nt, lds, s>>>(ord_tiles(A), A->bt, A->win_slot.p, val, x, A->win_rows.p, A->win_off.p, y);
+    kern<<<A->nbatches, nt, lds, s>>>(ord_tiles(A), A->bt, slot, val, pf, std::max(A->win_maxslots, 1), x,
+                                      A->win_rows.p, A->win_off.p, y);
     FL_LAUNCH_CHECK();
 }
 
-template <typename V, typename X, typename P, typename Y, bool SPLIT>
+template <typename V, typename X, typename P, typename Y, int MODE>
 static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *y, bool assign,
                       hipStream_t s) {
     constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group, as in WIN
-    const size_t lds1 = (size_t)A->bt * 64 * sizeof(P);
+    const PackFmt pf = A->pb_pf;
+    const uint16_t *lslot = MODE == kPacked ? nullptr : A->pb_lslot.p;
+    const size_t lds1 = ((size_t)A->bt * 64 + 64) * sizeof(P);  // + one spare slot per lane
     // 4 waves per 64 tiles of the batch (16 tiles per wave)
     const int nt = A->bt * 4;
-    auto k1 = (nt == 256) ? k_pb1<V, X, P, SPLIT, U, 256>
-              : (nt == 512) ? k_pb1<V, X, P, SPLIT, U, 512> : k_pb1<V, X, P, SPLIT, U, 1024>;
+    auto k1 = (nt == 256) ? k_pb1<V, X, P, MODE, U, 256>
+              : (nt == 512) ? k_pb1<V, X, P, MODE, U, 512> : k_pb1<V, X, P, MODE, U, 1024>;
     allow_lds(k1, std::max<size_t>(lds1, 8));
     const size_t lds2 = (size_t)A->pb_R * sizeof(double);
     allow_lds(k_pb2<P, Y, 8, true>, lds2);
@@ -960,7 +1102,7 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
     // phase 2 adds them into y (the first chunk assigns y when asked to)
     for (int c = 0; c < A->pb_chunks; ++c) {
         const int b0 = A->pb_chunk_batch[c], nbc = A->pb_chunk_batch[c + 1] - b0;
-        k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, A->pb_lslot.p, val, x, A->pb_run_off.p,
+        k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, lslot, val, pf, x, A->pb_run_off.p,
                                  A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
         FL_LAUNCH_CHECK();
         auto k2 = (assign && c == 0) ? k_pb2<P, Y, 8, true> : k_pb2<P, Y, 8, false>;
@@ -986,20 +1128,24 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
     }
     switch (kind) {
         case FL_VAL_F64:
-            if (win) launch_win<double, double, double, false>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
-            else launch_pb<double, double, double, double, false>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s);
+            if (win) launch_win<double, double, double, kPlain>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
+            else launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s);
             return FL_OK;
         case FL_VAL_F32:
-            if (win) launch_win<float, float, float, false>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<float, float, float, float, false>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s);
+            if (win) launch_win<float, float, float, kPlain>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
+            else launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16:
-            if (win) launch_win<uint16_t, float, float, false>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint16_t, float, float, float, false>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s);
+            if (win) launch_win<uint16_t, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
+            else launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s);
             return FL_OK;
         case FL_VAL_BF16X2:
-            if (win) launch_win<uint32_t, float, double, true>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
-            else launch_pb<uint32_t, float, float, double, true>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s);
+            if (win) launch_win<uint32_t, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
+            else launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s);
+            return FL_OK;
+        case FL_VAL_PACKED:
+            if (win) launch_win<uint32_t, float, float, kPacked>(A, A->win_pack.p, (const float *)x, (float *)y, assign, s);
+            else launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);

@@ -365,7 +365,7 @@ int fl_matrix_save(const fl_matrix *A, const char *path, void *stream) {
     h.max_col_nnz = A->max_col_nnz;
     h.device_layout = 1;
     h.value_kinds = (A->val32.p ? 1u << FL_VAL_F32 : 0u) | (A->bf16_hi.p ? 1u << FL_VAL_BF16 : 0u) |
-                    (A->bf16_pair.p ? 1u << FL_VAL_BF16X2 : 0u);
+                    (A->bf16_pair.p ? 1u << FL_VAL_BF16X2 : 0u) | (A->pack_wanted ? 1u << FL_VAL_PACKED : 0u);
     auto perm = to_host(A->perm, n, s), sortnnz = to_host(A->sortnnz, n, s);
     auto tp = to_host(A->tile_ptr, T + 1, s), tl = to_host(A->tile_len, T, s), tc = to_host(A->tile_colidx, T, s),
          tf = to_host(A->tile_format, T, s), cr = to_host(A->countrow, T + 1, s),
@@ -564,6 +564,7 @@ int fl_matrix_load(fl_matrix **out, const char *path, void *stream) {
         rc = row_install(A.get(), rp, rcol, rv, s);
         if (rc) return rc;
     }
+    if (h.value_kinds & (1u << FL_VAL_PACKED)) prepare_values(A.get(), FL_VAL_PACKED, s);  // after the schedules
     FL_HIP(hipStreamSynchronize(s));
     *out = A.release();
     return FL_OK;

@@ -201,8 +201,15 @@ __global__ void k_permute(const T *__restrict__ x, const int *__restrict__ perm,
 
 static unsigned sgrid(long long count) { return std::max(1u, std::min<unsigned>(grid_for(count), 65536u)); }
 
+void pack_schedules(fl_matrix *A, hipStream_t s);
+
 void prepare_values(fl_matrix *A, int kind, hipStream_t s) {
     const int nnz = A->nnz;
+    if (kind == FL_VAL_PACKED) {  // packed words of the prepared schedules (and of later ones)
+        A->pack_wanted = true;
+        pack_schedules(A, s);
+        return;
+    }
     if ((kind == FL_VAL_F32) && !A->val32.p) {
         A->val32.alloc(std::max(nnz, 1));
         if (nnz) k_to_f32<<<sgrid(nnz), kBlock, 0, s>>>(A->val64.p, A->val32.p, nnz);
@@ -267,7 +274,7 @@ static int refuse(const char *msg) {
 // Every condition under which a call would be refused, checked before anything is enqueued, so
 // that an invalid call leaves y as it was.
 static int spmv_check(const fl_matrix *A, int kernel, int kind, const void *work) {
-    if (kind < FL_VAL_F64 || kind > FL_VAL_BF16X2) {
+    if (kind < FL_VAL_F64 || kind > FL_VAL_PACKED) {
         set_error("fl_spmv: unknown value kind %d", kind);
         return FL_ERR_INVALID;
     }
@@ -277,6 +284,8 @@ static int spmv_check(const fl_matrix *A, int kernel, int kind, const void *work
     }
     if (A->tilenum == 0) return FL_OK;  // nothing to multiply: y = 0 (assign) or unchanged
     const bool f32_missing = kind == FL_VAL_F32 && !A->val32.p;
+    if (kind == FL_VAL_PACKED && (kernel == FL_KERNEL_DROPIN || kernel == FL_KERNEL_FAST))
+        return refuse("fl_spmv: FL_VAL_PACKED is a storage of the batched schedules (WIN, WINR, PB)");
     if (kernel == FL_KERNEL_DROPIN) {
         if (kind != FL_VAL_F64 && kind != FL_VAL_F32)
             return refuse("fl_spmv: the drop-in kernel supports FL_VAL_F64 and FL_VAL_F32");
@@ -302,6 +311,9 @@ static int spmv_check(const fl_matrix *A, int kernel, int kind, const void *work
     }
     if ((kind == FL_VAL_BF16 && !A->bf16_hi.p) || (kind == FL_VAL_BF16X2 && !A->bf16_pair.p))
         return refuse("fl_spmv: BF16 values not prepared");
+    if (kind == FL_VAL_PACKED && !(kernel == FL_KERNEL_PB ? A->pb_pack.p : A->win_pack.p))
+        return refuse("fl_spmv: no packed values for this schedule (fl_matrix_prepare_values(FL_VAL_PACKED) not "
+                      "called, or the slot bits and the matrix's exponent range leave fewer than 12 mantissa bits)");
     return FL_OK;
 }
 

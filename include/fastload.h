@@ -187,8 +187,16 @@ typedef struct {
 #define FL_VAL_F64 0    /* double values (drop-in FP64 mode), x/y double */
 #define FL_VAL_F32 1    /* float values, x/y float */
 #define FL_VAL_BF16 2   /* bfloat16 values, FP32 accumulation, x/y float */
-#define FL_VAL_BF16X2 3 /* split hi+lo bfloat16 values, FP32 accumulation of each half,
-                           per-row FP64 recombination; x float, y double */
+#define FL_VAL_BF16X2 3 /* split hi+lo bfloat16 values (hi = bf16(v), lo = bf16(v - hi)); x float,
+                           y double. hi*x and lo*x are FP32 products, recombined in FP64: per row
+                           by the FAST kernel (two FP32 row sums), per entry by WIN/WINR (FP64 LDS
+                           sums) and by PB (the FP64 sum of the pair rounded to one FP32 product,
+                           relative error 2^-24, before the FP64 band sums) */
+#define FL_VAL_PACKED 4 /* batched schedules (WIN/WINR/PB) only: each entry one 32-bit word
+                           holding its value and its 16-bit schedule slot, the value rounded
+                           to the mantissa bits left over (31 - slot bits - exponent-code bits,
+                           at least 12; refused otherwise) with an exponent code sized to the
+                           matrix's exponent range; x and y float, FP64 LDS sums */
 
 int fl_matrix_create(fl_matrix **A, int m, int n, int nnz, const int *d_colptr, const int *d_rowidx,
                      const double *d_val, int width, void *stream);
@@ -209,8 +217,14 @@ int fl_permute_x_f32(const fl_matrix *A, const float *d_x, float *d_sortx, void 
 
 /* Builds (once) the value copy of the given kind from the FP64 values. FL_VAL_BF16X2 keeps the
  * hi and lo arrays (FAST kernel) and the pair packed as one 32-bit word per entry (hi << 16 | lo,
- * read by the batched schedules). */
+ * read by the batched schedules). FL_VAL_PACKED builds the value + slot words of every batched
+ * schedule prepared so far and of those prepared later (each value rounded to nearest even, from
+ * FP64, to the mantissa bits left over; magnitudes below 2^-126 stored as zero). */
 int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
+/* The format of a schedule's FL_VAL_PACKED words: mantissa, exponent-code and slot bits (their
+ * sum plus the sign bit is at most 32). The relative rounding error of a value is at most
+ * 2^-(mantissa_bits+1). FL_ERR_INVALID when the schedule has no packed words. */
+int fl_matrix_pack_info(const fl_matrix *A, int kernel, int *mantissa_bits, int *exponent_bits, int *index_bits);
 
 /* SpMV kernels:
  *   FL_KERNEL_DROPIN: the reference per-tile algorithm (FastLoad___kernel,

@@ -475,3 +475,26 @@ void or_bf16_split(int count, const float *v, uint16_t *hi, uint16_t *lo)
         if (lo) lo[i] = bf16_rne(v[i] - bf16_to_f32(h));
     }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Packed value storage of the batched schedules (no reference counterpart): each value rounded
+ * to nearest even from FP64 to m explicit mantissa bits (1 <= m <= 23), magnitudes below the FP32
+ * normal range (exponent < -126) stored as zero, the sign of zero dropped. Returns the value the
+ * kernels decode, as a float (exact: m <= 23 and the exponent stays inside the FP32 range for the
+ * values the format accepts, exponent < 127).
+ * ---------------------------------------------------------------------------------------- */
+void or_pack_values(int count, const double *v, int m, float *out)
+{
+    for (int i = 0; i < count; ++i) {
+        uint64_t b;
+        memcpy(&b, &v[i], 8);
+        int e = (int)((b >> 52) & 0x7ff) - 1023;
+        if (e < -126) { out[i] = 0.0f; continue; }
+        uint64_t frac = b & ((1ull << 52) - 1), q = frac >> (52 - m);
+        uint64_t rem = frac & ((1ull << (52 - m)) - 1), half = 1ull << (51 - m);
+        if (rem > half || (rem == half && (q & 1))) ++q;
+        if (q >> m) { q = 0; ++e; }
+        double mag = ldexp(1.0 + (double)q / (double)(1ull << m), e);
+        out[i] = (float)((b >> 63) ? -mag : mag);
+    }
+}

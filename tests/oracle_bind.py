@@ -68,6 +68,7 @@ def lib():
                                      C.c_void_p, _f64p, _f64p]
         L.or_bf16_split.argtypes = [C.c_int, _f32p, _u16p, C.c_void_p]
         L.or_exclusive_scan.argtypes = [_i32p, C.c_int]
+        L.or_pack_values.argtypes = [C.c_int, _f64p, C.c_int, _f32p]
         _lib = L
     return _lib
 
@@ -152,6 +153,14 @@ def pipeline(m, n, colptr, rowidx, val, x, W=32):
     out = transform(n, nnz, sortnnz, tmp_row, tmp_val, W)
     out.update(perm=perm, sortnnz=sortnnz, tmp_row=tmp_row, tmp_val=tmp_val, sortx=sortx, m=m, n=n, nnz=nnz, W=W)
     return out
+
+
+def pack_values(v, m):
+    """Values rounded (nearest even) to m mantissa bits, as the packed storage decodes them."""
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros(max(v.size, 1), np.float32)
+    lib().or_pack_values(int(v.size), v if v.size else np.zeros(1), int(m), out)
+    return out[:v.size]
 
 
 def exclusive_scan(a):

@@ -14,7 +14,7 @@ from fastload_amd import DeviceArray
 
 pytestmark = pytest.mark.gpu
 
-TOL = {fl.VAL_F64: 1e-12, fl.VAL_F32: 1e-5, fl.VAL_BF16X2: 1e-4}
+TOL = {fl.VAL_F64: 1e-12, fl.VAL_F32: 1e-5, fl.VAL_BF16X2: 1e-4, fl.VAL_PACKED: 2.0 ** -13}
 LAYOUT_KEYS = ["perm", "sortnnz", "tile_ptr", "tile_len", "tile_colidx", "tile_format", "sortrowidx", "sortrowindex",
                "sortval", "countrow", "segmentoffset", "lvl_col", "lvl_base", "lvl_tall"]
 
@@ -60,8 +60,9 @@ def test_save_load_round_trip(gpu, tmp_path, name, mat, W, rows_window):
         else:
             M.prepare_schedule(fl.KERNEL_WIN, 128, 0)
             M.prepare_schedule(fl.KERNEL_PB, 128, 4096)
+            M.prepare(fl.VAL_PACKED)
             runs += [(fl.KERNEL_WIN, fl.VAL_F32), (fl.KERNEL_PB, fl.VAL_F32), (fl.KERNEL_PB, fl.VAL_F64),
-                     (fl.KERNEL_PB, fl.VAL_BF16X2)]
+                     (fl.KERNEL_PB, fl.VAL_BF16X2), (fl.KERNEL_PB, fl.VAL_PACKED), (fl.KERNEL_WIN, fl.VAL_PACKED)]
     path = tmp_path / f"{name}.flmx"
     M.save(path)
     L = fl.FastLoadMatrix.load(path)
@@ -74,7 +75,7 @@ def test_save_load_round_trip(gpu, tmp_path, name, mat, W, rows_window):
         y_loaded = _y(L, x, kernel, kind)
         y_orig = _y(M, x, kernel, kind)
         xv = x if kind == fl.VAL_F64 else x.astype(np.float32)
-        vv = v if kind == fl.VAL_F64 else v.astype(np.float32)
+        vv = v if kind in (fl.VAL_F64, fl.VAL_PACKED) else v.astype(np.float32)
         y_or, scale = ob.spmv_oracle(m, n, cp, ri, vv, xv)
         assert rel_err(y_loaded, y_or, scale) <= TOL[kind], (kernel, kind)
         assert rel_err(y_loaded, y_orig, scale) <= TOL[kind], (kernel, kind)

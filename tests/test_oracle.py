@@ -357,3 +357,39 @@ def test_fastload_cpu_f32_matches_serial_float32_loop(W):
             want[r] = np.float32(want[r] + np.float32(v[j] * x[cx + (j - s0) % ln]))
     got = ob.fastload_cpu_f32(t, v, x, m)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+# ------------------------------------------------------------------------- packed value rounding
+def test_pack_values_pinned_by_ieee_conversions():
+    """or_pack_values rounds FP64 to m mantissa bits, nearest even: with m = 23 it is the FP32
+    conversion and with m = 10 the FP16 conversion (numpy, round to nearest even) wherever those
+    formats hold the value as a normal number; below 2^-126 it flushes to zero."""
+    rng = np.random.default_rng(11)
+    v = np.concatenate([rng.standard_normal(20000) * np.exp2(rng.integers(-40, 40, 20000)),
+                        [0.0, -0.0, 1.0, -2.5, 1.5 * 2.0 ** -126, 2.0 ** -127, 3.0e38]])
+    assert np.array_equal(ob.pack_values(v, 23)[:-3], v.astype(np.float32)[:-3])
+    assert ob.pack_values(v, 23)[-3] == np.float32(1.5 * 2.0 ** -126) and ob.pack_values(v, 23)[-2] == 0.0
+    h = rng.standard_normal(20000) * np.exp2(rng.integers(-10, 12, 20000))
+    h = h[(np.abs(h) >= 2.0 ** -14) & (np.abs(h) < 65000)]  # FP16 normal numbers
+    assert h.size > 15000
+    assert np.array_equal(ob.pack_values(h, 10), h.astype(np.float16).astype(np.float32))
+
+
+@pytest.mark.parametrize("m", [12, 13, 17])
+def test_pack_values_ties_and_bounds(m):
+    """Brute checks for mantissa widths the schedules use: every result has at most m mantissa
+    bits, lies within half a unit of the last place, and exact halfway cases go to the even
+    neighbour (including the carry into the next binade)."""
+    rng = np.random.default_rng(m)
+    v = rng.standard_normal(5000) * np.exp2(rng.integers(-20, 20, 5000))
+    p = ob.pack_values(v, m).astype(np.float64)
+    ex = np.floor(np.log2(np.abs(v)))
+    ulp = np.exp2(ex - m)
+    assert np.all(np.abs(p - v) <= ulp / 2 * (1 + 1e-12))
+    assert np.all(np.mod(p / ulp, 1.0) == 0) or np.all(np.mod(np.abs(p) / np.exp2(np.floor(np.log2(np.abs(p))) - m), 1.0) == 0)
+    one = 1.0
+    tie_even = one + 2.0 ** -(m + 1)                 # halfway between 1 and 1 + 2^-m: goes down (even)
+    tie_odd = one + 2.0 ** -m + 2.0 ** -(m + 1)      # halfway above an odd mantissa: goes up
+    carry = 2.0 - 2.0 ** -(m + 1)                    # rounds up into the next binade
+    got = ob.pack_values(np.array([tie_even, tie_odd, carry, -tie_odd]), m).astype(np.float64)
+    assert got.tolist() == [1.0, 1.0 + 2.0 ** -(m - 1), 2.0, -(1.0 + 2.0 ** -(m - 1))]
