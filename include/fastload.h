@@ -273,6 +273,16 @@ int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, 
 int fl_matrix_save(const fl_matrix *A, const char *path, void *stream);
 int fl_matrix_load(fl_matrix **A, const char *path, void *stream);
 
+/* y = A * sortx like fl_spmv_assign, allocating nothing: no reference counterpart. For callers
+ * that cannot allocate device memory during the call (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The PB schedule's product array and the FAST
+ * kernel's split-BF16 work buffer live in a static 64 MiB device array per GPU instead of memory
+ * the library or the caller allocates; a call needing more (PB: the matrix's product ring, up to
+ * nnz products of 4 bytes, 8 in FP64; FAST split BF16: 2*m floats) returns FL_ERR_INVALID before
+ * anything is enqueued. Schedules and value copies must be prepared beforehand (those calls do
+ * allocate). All calls on one GPU share the array: calls that overlap on one GPU race. */
+int fl_spmv_assign_noalloc(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *stream);
+
 /* Builds (once per parameter set) the preprocessing of a batched schedule on a 64-wide matrix:
  *   kernel FL_KERNEL_WIN / FL_KERNEL_WINR: batch_tiles in {64, 128, 256} (LDS <= batch_tiles*512 bytes);
  *   kernel FL_KERNEL_PB : batch_tiles as above, band_rows in [1024, 16384] (LDS = band_rows*8 bytes).
