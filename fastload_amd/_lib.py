@@ -70,6 +70,8 @@ SIGNATURES = {
     "fl_spmv_assign": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
     "fl_spmv_assign_noalloc": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),
     "fl_matrix_prepare_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "fl_matrix_prepare_hyb": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "fl_matrix_hyb_info": (C.c_int, [vp, i32p, C.POINTER(C.c_longlong)]),
     "fl_matrix_prepare_rows": (C.c_int, [vp, vp]),
     "fl_spmm": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
     "fl_batch_assemble": (C.c_int, [C.c_int, vp, vp, vp, vp, vp, i32p, i32p, C.POINTER(i32p), C.POINTER(i32p),

@@ -16,7 +16,7 @@ from ._lib import FastLoadError, MatrixInfo, SlideMatrix, check, i32p, f64p, lib
 from .device import DeviceArray
 
 VAL_F64, VAL_F32, VAL_BF16, VAL_BF16X2, VAL_PACKED = 0, 1, 2, 3, 4
-KERNEL_DROPIN, KERNEL_FAST, KERNEL_WIN, KERNEL_PB, KERNEL_WINR = 0, 1, 2, 3, 4
+KERNEL_DROPIN, KERNEL_FAST, KERNEL_WIN, KERNEL_PB, KERNEL_WINR, KERNEL_HYB = 0, 1, 2, 3, 4, 5
 
 GEN_SYMGEOM, GEN_ZIPF, GEN_STENCIL27, GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF_DENSE = range(7)
 VEC_NORMAL, VEC_UNIFORM, VEC_MOD10 = range(3)
@@ -305,6 +305,19 @@ class FastLoadMatrix:
     def prepare_schedule(self, kernel: int, batch_tiles: int = 256, band_rows: int = 16384, stream=None):
         check(lib.fl_matrix_prepare_schedule(self.handle, kernel, batch_tiles, band_rows,
                                              stream.ptr if stream is not None else None), "fl_matrix_prepare_schedule")
+
+    def prepare_hyb(self, heavy_cols: int | None = None, batch_tiles: int = 128, band_rows: int = 16384, stream=None):
+        """The HYB schedule: the last `heavy_cols` sorted columns (default min(n, 32768)) from LDS,
+        the others through a PB sub-matrix of (batch_tiles, band_rows)."""
+        h = min(self.n, 32768) if heavy_cols is None else heavy_cols
+        check(lib.fl_matrix_prepare_hyb(self.handle, h, batch_tiles, band_rows,
+                                        stream.ptr if stream is not None else None), "fl_matrix_prepare_hyb")
+
+    def hyb_info(self):
+        """(heavy columns, heavy entries) of the prepared HYB schedule."""
+        h, z = C.c_int(), C.c_longlong()
+        check(lib.fl_matrix_hyb_info(self.handle, C.byref(h), C.byref(z)), "fl_matrix_hyb_info")
+        return h.value, z.value
 
     def prepare_rows(self, stream=None):
         check(lib.fl_matrix_prepare_rows(self.handle, stream.ptr if stream is not None else None),

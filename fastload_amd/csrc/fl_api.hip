@@ -548,9 +548,13 @@ int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream) {
 
 int fl_matrix_pack_info(const fl_matrix *A, int kernel, int *mantissa_bits, int *exponent_bits, int *index_bits) {
     if (!A) return invalid("fl_matrix_pack_info: NULL");
+    if (kernel == FL_KERNEL_HYB) {  // the PB words of the HYB schedule's sub-matrix
+        if (!A->hyb_ready) return invalid("fl_matrix_pack_info: HYB schedule not prepared");
+        return fl_matrix_pack_info(A->hyb_light.get(), FL_KERNEL_PB, mantissa_bits, exponent_bits, index_bits);
+    }
     const bool pb = kernel == FL_KERNEL_PB;
     if (!pb && kernel != FL_KERNEL_WIN && kernel != FL_KERNEL_WINR)
-        return invalid("fl_matrix_pack_info: kernel must be WIN, WINR or PB");
+        return invalid("fl_matrix_pack_info: kernel must be WIN, WINR, PB or HYB");
     if (!(pb ? A->pb_pack.p : A->win_pack.p) || (!pb && A->ord_kind != (kernel == FL_KERNEL_WINR ? 1 : 0)))
         return invalid("fl_matrix_pack_info: no packed values for this schedule");
     const PackFmt &f = pb ? A->pb_pf : A->win_pf;
@@ -560,11 +564,52 @@ int fl_matrix_pack_info(const fl_matrix *A, int kernel, int *mantissa_bits, int 
     return FL_OK;
 }
 
+static int hyb_args(const fl_matrix *A, int heavy_cols, int batch_tiles, int band_rows, const char *who) {
+    if (heavy_cols < 0 || heavy_cols > std::min(A->n, kHybMaxH32)) {
+        set_error("%s: heavy_cols must be in [0, min(n, %d)]", who, kHybMaxH32);
+        return FL_ERR_INVALID;
+    }
+    if (batch_tiles != 64 && batch_tiles != 128 && batch_tiles != 256) {
+        set_error("%s: batch_tiles must be 64, 128 or 256", who);
+        return FL_ERR_INVALID;
+    }
+    if (band_rows < 1024 || band_rows > 16384) {
+        set_error("%s: band_rows must be in [1024, 16384]", who);
+        return FL_ERR_INVALID;
+    }
+    return FL_OK;
+}
+
+int fl_matrix_prepare_hyb(fl_matrix *A, int heavy_cols, int batch_tiles, int band_rows, void *stream) {
+    FL_API_BEGIN
+    if (!A) return invalid("fl_matrix_prepare_hyb: NULL");
+    const int rc = hyb_args(A, heavy_cols, batch_tiles, band_rows, "fl_matrix_prepare_hyb");
+    if (rc) return rc;
+    hyb_prepare(A, heavy_cols, batch_tiles, band_rows, (hipStream_t)stream);
+    return FL_OK;
+    FL_API_END
+}
+
+int fl_matrix_hyb_info(const fl_matrix *A, int *heavy_cols, long long *heavy_nnz) {
+    if (!A) return invalid("fl_matrix_hyb_info: NULL");
+    if (!A->hyb_ready) return invalid("fl_matrix_hyb_info: HYB schedule not prepared");
+    if (heavy_cols) *heavy_cols = A->hyb_H;
+    if (heavy_nnz) *heavy_nnz = A->hyb_nnz;
+    return FL_OK;
+}
+
 int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int band_rows, void *stream) {
     FL_API_BEGIN
     if (!A) return invalid("fl_matrix_prepare_schedule: NULL");
+    if (kernel == FL_KERNEL_HYB) {
+        const int H = std::min(A->n, 32768);
+        const int rc = hyb_args(A, H, batch_tiles, band_rows, "fl_matrix_prepare_schedule");
+        if (rc) return rc;
+        hyb_prepare(A, H, batch_tiles, band_rows, (hipStream_t)stream);
+        return FL_OK;
+    }
     if (kernel != FL_KERNEL_WIN && kernel != FL_KERNEL_WINR && kernel != FL_KERNEL_PB)
-        return invalid("fl_matrix_prepare_schedule: kernel must be WIN, WINR or PB");
+        return invalid("fl_matrix_prepare_schedule: kernel must be WIN, WINR, PB or HYB");
     if (A->W != 64) return invalid("fl_matrix_prepare_schedule: needs a 64-wide tile layout");
     if (batch_tiles != 64 && batch_tiles != 128 && batch_tiles != 256)
         return invalid("fl_matrix_prepare_schedule: batch_tiles must be 64, 128 or 256");

@@ -1,6 +1,7 @@
 // Device-resident FastLoad matrix (internal layout; the public handle is opaque).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "fl_common.h"
@@ -102,6 +103,28 @@ struct fl_matrix {
     fl::DevBuf<int> row_ptr, row_col;
     fl::DevBuf<float> row_val32;
     fl::DevBuf<double> row_val64;
+
+    // HYB: the last hyb_H sorted columns (the longest: the column sort is ascending) are
+    // multiplied with x held in LDS, row by row; the other columns form a sub-matrix (hyb_light,
+    // its sorted column i = sorted column i here) with its own PB schedule, whose first phase 2
+    // adds the heavy part's y. The rows are ordered by heavy entry count, longest first, inside
+    // windows of kHybWin rows (hyb_iperm); slice s = sorted positions [64 s, 64 s + 64) is stored
+    // as hyb_np[s] pair-slots of 64 lanes: slot (q, lane) holds the lane's row's entries 2q and
+    // 2q + 1 (value pair + two 16-bit columns in one word), the rows shorter than the slice's
+    // longest padded with value 0 and column H (a zero in LDS).
+    bool hyb_ready = false;
+    int hyb_H = 0, hyb_bt = 0, hyb_R = 0, hyb_nslice = 0, hyb_grid = 0;
+    long long hyb_nnz = 0;            // heavy entries
+    long long hyb_slots = 0;          // pair-slots of all slices (64 per pair row)
+    std::unique_ptr<fl_matrix> hyb_light;
+    fl::DevBuf<long long> hyb_sb;     // [nslice + 1] first slot of every slice
+    fl::DevBuf<int> hyb_np;           // [nslice] pair rows of every slice (32-bit: scalar loads)
+    fl::DevBuf<uint16_t> hyb_iperm;   // [m] sorted position of each row inside its window
+    fl::DevBuf<uint32_t> hyb_col2;    // [hyb_slots + pad] two columns per slot
+    fl::DevBuf<double> hyb_val64;     // [2 (hyb_slots + pad)] two values per slot
+    fl::DevBuf<float> hyb_val32;      // built with the FP32 values
+    fl::DevBuf<float> hyb_ys32;       // [m] heavy part of y by sorted position (first use)
+    fl::DevBuf<double> hyb_ys64;
 };
 
 namespace fl {
@@ -145,5 +168,21 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
 // kind's product type, 16-byte aligned); nullptr = the matrix's own, allocated on first use.
 int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s,
          void *pb_bins = nullptr);
+
+// The WIN/WINR/PB launches (fl_sched.hip). fold_ys/fold_iperm (PB only): the first chunk's phase 2
+// also adds fold_ys[(r & ~(kHybWin - 1)) + fold_iperm[r]] to row r (the HYB schedule's heavy part).
+int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
+                 void *pb_bins, const void *fold_ys = nullptr, const uint16_t *fold_iperm = nullptr);
+
+// HYB schedule (fl_hyb.hip). Heavy x is staged in LDS: at most kHybMaxH32 FP32 or kHybMaxH64
+// FP64 columns.
+constexpr int kHybMaxH32 = 38912, kHybMaxH64 = 19456;
+constexpr int kHybWinLog = 14, kHybWin = 1 << kHybWinLog;  // rows of a sorting window
+void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s);
+void hyb_values(fl_matrix *A, int kind, hipStream_t s);
+int hyb_check(const fl_matrix *A, int kind);
+// scratch: nullptr (the matrix's own buffers, allocated on first use) or hyb_scratch_bytes bytes
+size_t hyb_scratch_bytes(const fl_matrix *A, int kind);
+int spmv_hyb(fl_matrix *A, int kind, const void *x, void *y, bool assign, hipStream_t s, void *scratch);
 
 }  // namespace fl

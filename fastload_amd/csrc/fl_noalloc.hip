@@ -2,17 +2,19 @@
 // memory during the call (code under HIP graph capture, tight serving loops, harnesses that forbid
 // hipMalloc). No reference counterpart.
 //
-// fl_spmv_assign needs device memory beyond its inputs and outputs in two places: the PB
+// fl_spmv_assign needs device memory beyond its inputs and outputs in three places: the PB
 // schedule's product array (pb_ring products, allocated by the library on the matrix's first PB
-// call) and the FAST kernel's split-BF16 work buffer (2*m floats, passed by the caller). Here both
-// live in one file-scope device array, `scratch`, of kScratchBytes, which every GPU the library
-// runs on holds until the process ends. Its device address is looked up once per device (the
+// call), the HYB schedule's heavy-part y (m values) and its sub-matrix's product array, and the
+// FAST kernel's split-BF16 work buffer (2*m floats, passed by the caller). Here all of them live
+// in one file-scope device array, `scratch`, of kScratchBytes, which every GPU the library runs
+// on holds until the process ends. Its device address is looked up once per device (the
 // first call on that device) with hipGetSymbolAddress and kept; host code never hands the array's
 // own name to a kernel (on the host that name is only a stand-in, not a device address).
 //
 // Every call on a GPU shares that GPU's array and the array keeps the last call's contents: the
-// PB products are all written by phase 1 before phase 2 reads them, and the split-BF16 partial
-// sums are cleared by a fill kernel, so nothing depends on what an earlier call left. Calls that
+// PB products are all written by phase 1 before phase 2 reads them, the HYB heavy kernel writes
+// every row's value before the sub-matrix's phase 2 reads it, and the split-BF16 partial sums are
+// cleared by a fill kernel, so nothing depends on what an earlier call left. Calls that
 // overlap on one GPU (several streams, several host threads) race on the array.
 #include <mutex>
 #include <vector>
@@ -62,6 +64,8 @@ int fl_spmv_assign_noalloc(fl_matrix *A, int kernel, int val_kind, const void *d
     size_t need = 0;
     if (kernel == FL_KERNEL_PB && A->pb_ready)
         need = (size_t)A->pb_ring * (val_kind == FL_VAL_F64 ? sizeof(double) : sizeof(float));
+    else if (kernel == FL_KERNEL_HYB && A->hyb_ready)
+        need = hyb_scratch_bytes(A, val_kind);
     else if (kernel == FL_KERNEL_FAST && val_kind == FL_VAL_BF16X2)
         need = sizeof(float) * 2 * (size_t)A->m;
     if (need > kScratchBytes) {

@@ -977,11 +977,13 @@ struct PbVec<double> {
 
 // ASSIGN: y = A x (every row of the band is written, so y needs no clearing); else y += A x.
 // band_start points at one chunk's band table; the product at band-major position i is
-// bins[i - off] (off a multiple of 4), its row rowlocal[i].
-template <typename P, typename Y, int U, bool ASSIGN>
+// bins[i - off] (off a multiple of 4), its row rowlocal[i]. FOLD: row r also gets
+// fys[(r & ~(kHybWin - 1)) + fiperm[r]] (a part of y kept in another row order, added in FP64).
+template <typename P, typename Y, int U, bool ASSIGN, bool FOLD>
 __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
                                                    const P *__restrict__ bins, int off,
-                                                   const uint16_t *__restrict__ rowlocal, Y *__restrict__ y) {
+                                                   const uint16_t *__restrict__ rowlocal, Y *__restrict__ y,
+                                                   const Y *__restrict__ fys, const uint16_t *__restrict__ fiperm) {
     extern __shared__ double acc[];
     using T = PbVec<P>;
     constexpr int N = T::N;
@@ -1035,8 +1037,13 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
     __syncthreads();
     const int r0 = b * R, rows = min(R, m - r0);
     for (int r = threadIdx.x; r < rows; r += kBigBlock) {
-        if constexpr (ASSIGN) y[r0 + r] = (Y)acc[r];
-        else y[r0 + r] += (Y)acc[r];
+        double v = acc[r];
+        if constexpr (FOLD) {
+            const long long g = (long long)r0 + r;
+            v += (double)fys[(g & ~(long long)(kHybWin - 1)) + fiperm[g]];
+        }
+        if constexpr (ASSIGN) y[r0 + r] = (Y)v;
+        else y[r0 + r] += (Y)v;
     }
 }
 
@@ -1085,7 +1092,7 @@ static void launch_win(const fl_matrix *A, const V *val, const X *x, Y *y, bool 
 
 template <typename V, typename X, typename P, typename Y, int MODE>
 static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *y, bool assign,
-                      hipStream_t s) {
+                      hipStream_t s, const Y *fys, const uint16_t *fiperm) {
     constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group, as in WIN
     const PackFmt pf = A->pb_pf;
     const uint16_t *lslot = MODE == kPacked ? nullptr : A->pb_lslot.p;
@@ -1096,8 +1103,12 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
               : (nt == 512) ? k_pb1<V, X, P, MODE, U, 512> : k_pb1<V, X, P, MODE, U, 1024>;
     allow_lds(k1, std::max<size_t>(lds1, 8));
     const size_t lds2 = (size_t)A->pb_R * sizeof(double);
-    allow_lds(k_pb2<P, Y, 8, true>, lds2);
-    allow_lds(k_pb2<P, Y, 8, false>, lds2);
+    allow_lds(k_pb2<P, Y, 8, true, false>, lds2);
+    allow_lds(k_pb2<P, Y, 8, false, false>, lds2);
+    if (fys) {
+        allow_lds(k_pb2<P, Y, 8, true, true>, lds2);
+        allow_lds(k_pb2<P, Y, 8, false, true>, lds2);
+    }
     // chunk by chunk: phase 1 writes the chunk's products to the front of the product array,
     // phase 2 adds them into y (the first chunk assigns y when asked to)
     for (int c = 0; c < A->pb_chunks; ++c) {
@@ -1105,9 +1116,11 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
         k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, lslot, val, pf, x, A->pb_run_off.p,
                                  A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
         FL_LAUNCH_CHECK();
-        auto k2 = (assign && c == 0) ? k_pb2<P, Y, 8, true> : k_pb2<P, Y, 8, false>;
+        const bool fold = fys && c == 0;  // the first chunk's phase 2 writes every row once
+        auto k2 = (assign && c == 0) ? (fold ? k_pb2<P, Y, 8, true, true> : k_pb2<P, Y, 8, true, false>)
+                                     : (fold ? k_pb2<P, Y, 8, false, true> : k_pb2<P, Y, 8, false, false>);
         k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p + (size_t)c * A->nbands, bins,
-                                              A->pb_chunk_off[c], A->pb_rowlocal.p, y);
+                                              A->pb_chunk_off[c], A->pb_rowlocal.p, y, fys, fiperm);
         FL_LAUNCH_CHECK();
     }
 }
@@ -1116,7 +1129,7 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
 // its products to pb_bins when given (the caller guarantees pb_ring entries), else to the
 // matrix's own product array, allocated on first use.
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
-                 void *pb_bins) {
+                 void *pb_bins, const void *fold_ys, const uint16_t *fold_iperm) {
     const bool win = (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR);
     double *bins64 = (double *)pb_bins;
     float *bins32 = (float *)pb_bins;
@@ -1129,23 +1142,23 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
     switch (kind) {
         case FL_VAL_F64:
             if (win) launch_win<double, double, double, kPlain>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
-            else launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s);
+            else launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s, (const double *)fold_ys, fold_iperm);
             return FL_OK;
         case FL_VAL_F32:
             if (win) launch_win<float, float, float, kPlain>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s);
+            else launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s, (const float *)fold_ys, fold_iperm);
             return FL_OK;
         case FL_VAL_BF16:
             if (win) launch_win<uint16_t, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s);
+            else launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s, (const float *)fold_ys, fold_iperm);
             return FL_OK;
         case FL_VAL_BF16X2:
             if (win) launch_win<uint32_t, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
-            else launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s);
+            else launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s, (const double *)fold_ys, fold_iperm);
             return FL_OK;
         case FL_VAL_PACKED:
             if (win) launch_win<uint32_t, float, float, kPacked>(A, A->win_pack.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s);
+            else launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s, (const float *)fold_ys, fold_iperm);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);

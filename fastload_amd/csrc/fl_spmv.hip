@@ -208,6 +208,7 @@ void prepare_values(fl_matrix *A, int kind, hipStream_t s) {
     if (kind == FL_VAL_PACKED) {  // packed words of the prepared schedules (and of later ones)
         A->pack_wanted = true;
         pack_schedules(A, s);
+        if (A->hyb_light) hyb_values(A, kind, s);
         return;
     }
     if ((kind == FL_VAL_F32) && !A->val32.p) {
@@ -226,6 +227,7 @@ void prepare_values(fl_matrix *A, int kind, hipStream_t s) {
         if (nnz) k_pack_pair<<<sgrid(nnz), kBlock, 0, s>>>(A->bf16_hi.p, A->bf16_lo.p, A->bf16_pair.p, nnz);
         FL_LAUNCH_CHECK();
     }
+    if (A->hyb_light) hyb_values(A, kind, s);
     FL_HIP(hipStreamSynchronize(s));
 }
 
@@ -262,8 +264,6 @@ static void launch_fast(const fl_matrix *A, const V *val, const V *val_lo, const
     FL_LAUNCH_CHECK();
 }
 
-int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
-                 void *pb_bins);
 void zero_fill(void *d, size_t bytes, hipStream_t s);
 
 static int refuse(const char *msg) {
@@ -278,11 +278,12 @@ static int spmv_check(const fl_matrix *A, int kernel, int kind, const void *work
         set_error("fl_spmv: unknown value kind %d", kind);
         return FL_ERR_INVALID;
     }
-    if (kernel < FL_KERNEL_DROPIN || kernel > FL_KERNEL_WINR) {
+    if (kernel < FL_KERNEL_DROPIN || kernel > FL_KERNEL_HYB) {
         set_error("fl_spmv: unknown kernel %d", kernel);
         return FL_ERR_INVALID;
     }
     if (A->tilenum == 0) return FL_OK;  // nothing to multiply: y = 0 (assign) or unchanged
+    if (kernel == FL_KERNEL_HYB) return hyb_check(A, kind);
     const bool f32_missing = kind == FL_VAL_F32 && !A->val32.p;
     if (kind == FL_VAL_PACKED && (kernel == FL_KERNEL_DROPIN || kernel == FL_KERNEL_FAST))
         return refuse("fl_spmv: FL_VAL_PACKED is a storage of the batched schedules (WIN, WINR, PB)");
@@ -325,8 +326,10 @@ int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work,
     const int rc = spmv_check(A, kernel, kind, work);
     if (rc) return rc;
     const size_t ybytes = (size_t)A->m * ((kind == FL_VAL_F64 || kind == FL_VAL_BF16X2) ? 8 : 4);
-    if (assign && (A->tilenum == 0 || kernel != FL_KERNEL_PB)) zero_fill(y, ybytes, s);
+    // PB and HYB write every row once when assigning; the other kernels accumulate
+    if (assign && (A->tilenum == 0 || (kernel != FL_KERNEL_PB && kernel != FL_KERNEL_HYB))) zero_fill(y, ybytes, s);
     if (A->tilenum == 0) return FL_OK;
+    if (kernel == FL_KERNEL_HYB) return spmv_hyb(A, kind, x, y, assign, s, pb_bins);
     if (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR || kernel == FL_KERNEL_PB)
         return spmv_batched(A, kernel, kind, x, y, assign, s, pb_bins);
     if (kernel == FL_KERNEL_DROPIN) {

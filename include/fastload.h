@@ -223,7 +223,8 @@ int fl_permute_x_f32(const fl_matrix *A, const float *d_x, float *d_sortx, void 
 int fl_matrix_prepare_values(fl_matrix *A, int val_kind, void *stream);
 /* The format of a schedule's FL_VAL_PACKED words: mantissa, exponent-code and slot bits (their
  * sum plus the sign bit is at most 32). The relative rounding error of a value is at most
- * 2^-(mantissa_bits+1). FL_ERR_INVALID when the schedule has no packed words. */
+ * 2^-(mantissa_bits+1). FL_KERNEL_HYB: the words of its PB sub-matrix (the heavy part keeps FP32
+ * values). FL_ERR_INVALID when the schedule has no packed words. */
 int fl_matrix_pack_info(const fl_matrix *A, int kernel, int *mantissa_bits, int *exponent_bits, int *index_bits);
 
 /* SpMV kernels:
@@ -250,6 +251,13 @@ int fl_matrix_pack_info(const fl_matrix *A, int kernel, int *mantissa_bits, int 
  *     instead of by column: on stencils and bands most rows then belong to a single batch and are
  *     written without an atomic. */
 #define FL_KERNEL_WINR 4
+/*   FL_KERNEL_HYB (no reference counterpart): the longest sorted columns ("heavy", at most 38912,
+ *     or 19456 in FP64) multiply x staged once per workgroup in LDS, row by row in jagged-diagonal
+ *     slices of 64 rows (value + 16-bit column per entry, no index traffic for y); the other
+ *     columns run the PB schedule as their own sub-matrix, whose phase 2 also adds the heavy
+ *     part's y. For matrices whose few longest columns hold most entries (power laws). Value kinds
+ *     FL_VAL_F64, FL_VAL_F32 and FL_VAL_PACKED (the light part's PB words; heavy values FP32). */
+#define FL_KERNEL_HYB 5
 
 /* y += A * sortx (y must be zeroed by the caller for y = A*x, as src/main.cu:288-313 does).
  * FL_KERNEL_WIN/FL_KERNEL_PB need fl_matrix_prepare_schedule first; they write FL_VAL_BF16X2
@@ -289,6 +297,15 @@ int fl_spmv_assign_noalloc(fl_matrix *A, int kernel, int val_kind, const void *d
  * The PB schedule keeps a product buffer of nnz floats (doubles in FP64 mode) allocated on its
  * first use. */
 int fl_matrix_prepare_schedule(fl_matrix *A, int kernel, int batch_tiles, int band_rows, void *stream);
+/* Builds the FL_KERNEL_HYB schedule (no reference counterpart): the last heavy_cols sorted columns
+ * in [0, min(n, 38912)] become the heavy part, the others a sub-matrix with the PB schedule of
+ * batch_tiles and band_rows (as above). fl_matrix_prepare_schedule(A, FL_KERNEL_HYB, ...) uses
+ * heavy_cols = min(n, 32768). Value copies prepared before or after are built for both parts.
+ * Extra memory: the heavy entries (8 + 2 bytes each, + 4 with FP32 values), four arrays of m
+ * or m/64 entries, the sub-matrix and its PB schedule. */
+int fl_matrix_prepare_hyb(fl_matrix *A, int heavy_cols, int batch_tiles, int band_rows, void *stream);
+/* The prepared HYB split: heavy columns and heavy entries (the rest are in the PB sub-matrix). */
+int fl_matrix_hyb_info(const fl_matrix *A, int *heavy_cols, long long *heavy_nnz);
 
 /* ------------------------------------ multiple right-hand sides (no reference counterpart)
  * Y (m x k, row-major) = A * X  (accumulate = 0)  or  Y += A * X  (accumulate = 1), where X is

@@ -17,13 +17,16 @@ pytestmark = pytest.mark.gpu
 TOL = {fl.VAL_F64: 1e-12, fl.VAL_F32: 1e-5, fl.VAL_BF16X2: 1e-4, fl.VAL_PACKED: 2.0 ** -13}
 CASES = [(fl.KERNEL_PB, fl.VAL_F32), (fl.KERNEL_PB, fl.VAL_F64), (fl.KERNEL_PB, fl.VAL_BF16X2),
          (fl.KERNEL_PB, fl.VAL_PACKED), (fl.KERNEL_FAST, fl.VAL_BF16X2), (fl.KERNEL_WIN, fl.VAL_F32),
-         (fl.KERNEL_DROPIN, fl.VAL_F64)]
+         (fl.KERNEL_DROPIN, fl.VAL_F64), (fl.KERNEL_HYB, fl.VAL_F32), (fl.KERNEL_HYB, fl.VAL_F64),
+         (fl.KERNEL_HYB, fl.VAL_PACKED)]
 
 
 def _prepared(m, n, cp, ri, v, kernel, kind):
     M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
     if kernel in (fl.KERNEL_PB, fl.KERNEL_WIN):
         M.prepare_schedule(kernel, 128, 8192)
+    if kernel == fl.KERNEL_HYB:
+        M.prepare_hyb(2048, 128, 8192)
     if kind != fl.VAL_F64:
         M.prepare(kind)
     return M
@@ -48,7 +51,7 @@ def _run_pair(m, n, cp, ri, v, x, kernel, kind):
 
 
 @pytest.mark.parametrize("kernel,kind", CASES, ids=["pb-f32", "pb-f64", "pb-bf16x2", "pb-packed", "fast-bf16x2",
-                                                    "win-f32", "dropin-f64"])
+                                                    "win-f32", "dropin-f64", "hyb-f32", "hyb-f64", "hyb-packed"])
 def test_noalloc_matches_assign_two_calls(gpu, kernel, kind):
     rng = np.random.default_rng(kernel * 10 + kind)
     big = fl.gen_matrix(fl.GEN_ZIPF, 31, 300000, 300000, (1.8, 4000))
