@@ -75,6 +75,9 @@ VARIANTS = {
     "pb8k_f32": (64, 3, 1, 4, 256, 8192),
     "pb4k_f32": (64, 3, 1, 4, 256, 4096),
     "pb128_8k_f32": (64, 3, 1, 4, 128, 8192),
+    # HYB (kernel 5): heaviest 32768 columns from LDS (FP32 values), the rest PB 128 x 16K
+    "hyb_f32": (64, 5, 1, 4, 128, 16384),
+    "hyb_packed": (64, 5, 4, 4, 128, 16384),
 }
 # value storage -> (name, value bytes beside a 4-byte row index); packed words hold the value and
 # the schedule's slot in 4 bytes, so they count 0 + 4
@@ -85,9 +88,11 @@ def learned_variants(sched, names=("f32", "bf16x2", "bf16", "packed")):
     """Value-storage variants of the schedule the selector chose: FP32, split hi+lo BF16, BF16 and
     packed value + slot words (those of `names` that the schedule supports without a work
     buffer)."""
-    kinds = [1, 3, 2, 4] if sched.kernel in (2, 3, 4) else ([1, 2] if sched.kernel >= 0 else [1])
-    return {f"learned_{KIND_SVAL[k][0]}": (sched.width, sched.kernel, k, KIND_SVAL[k][1], sched.batch_tiles,
-                                           sched.band_rows) for k in kinds if KIND_SVAL[k][0] in names}
+    kinds = ([1, 3, 2, 4] if sched.kernel in (2, 3, 4) else [1, 4] if sched.kernel == 5
+             else ([1, 2] if sched.kernel >= 0 else [1]))
+    # HYB keeps FP32 values for its heavy part (most entries): 4 value bytes in both kinds
+    return {f"learned_{KIND_SVAL[k][0]}": (sched.width, sched.kernel, k, 4 if sched.kernel == 5 else KIND_SVAL[k][1],
+                                           sched.batch_tiles, sched.band_rows) for k in kinds if KIND_SVAL[k][0] in names}
 
 
 def profiled_traffic(schedule: str, kind: str, workload: str):
@@ -140,8 +145,8 @@ def select_schedule(fl, m, n, dcp, dri, dv, top=None, force=None):
     skips the selection, for profiling runs whose timings the profiler skews."""
     from fastload_amd import selector as sel
     if force:
-        c = next(i for i, sc in enumerate(sel.SCHEDULES) if sc.name == force)
-        return sel.SCHEDULES[c], dict(cls=c, schedule=force, forced=True)
+        c = next(i for i, sc in enumerate(sel.ALL_SCHEDULES) if sc.name == force)
+        return sel.ALL_SCHEDULES[c], dict(cls=c, schedule=force, forced=True)
     s = sel.Selector()
     fl.synchronize()
     t0 = time.perf_counter()
@@ -150,11 +155,12 @@ def select_schedule(fl, m, n, dcp, dri, dv, top=None, force=None):
     t0 = time.perf_counter()
     c, times, _ = s.choose_refined(m, n, dcp, dri, dv, top=top)
     ms = (time.perf_counter() - t0) * 1e3
-    return sel.SCHEDULES[c], dict(cls=c, schedule=sel.SCHEDULES[c].name, network_choice=sel.SCHEDULES[c1].name,
-                                  network_prob=round(p, 4), select_ms=round(ms1, 3), tuned_candidates=len(times),
-                                  tuned_ms={sel.SCHEDULES[k].name: (round(v, 4) if v is not None else None)
-                                            for k, v in times.items()},
-                                  tuning_s=round(ms / 1e3, 3), trained=s.mlp is not None)
+    return sel.ALL_SCHEDULES[c], dict(cls=c, schedule=sel.ALL_SCHEDULES[c].name,
+                                      network_choice=sel.SCHEDULES[c1].name, network_prob=round(p, 4),
+                                      select_ms=round(ms1, 3), tuned_candidates=len(times),
+                                      tuned_ms={sel.ALL_SCHEDULES[k].name: (round(v, 4) if v is not None else None)
+                                                for k, v in times.items()},
+                                      tuning_s=round(ms / 1e3, 3), trained=s.mlp is not None)
 
 
 def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup, flush=None):
@@ -199,14 +205,16 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup, flush
                 M.prepare(kind)
             if kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
                 M.prepare_schedule(kernel, bt, max(R, 1024))
+            elif kernel == fl.KERNEL_HYB:
+                M.prepare_hyb(None, bt, R)
             ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
-            assign = kernel in (fl.KERNEL_PB, fl.KERNEL_WIN, fl.KERNEL_WINR)
+            assign = kernel in (fl.KERNEL_PB, fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_HYB)
 
             def run(M=M, kernel=kernel, kind=kind, dsx=sx[(W, s_x)], ydev=ydev, assign=assign):
-                # y = A x: fl_spmv_assign for the batched schedules (PB writes every row in its
-                # second phase; WIN/WINR clear y with a fill kernel inside the call, then store the
-                # rows one batch owns); the drop-in kernels accumulate into a y cleared by a fill
-                # kernel outside the kernel events
+                # y = A x: fl_spmv_assign for the batched schedules (PB and HYB write every row in
+                # PB's second phase; WIN/WINR clear y with a fill kernel inside the call, then store
+                # the rows one batch owns); the drop-in kernels accumulate into a y cleared by a
+                # fill kernel outside the kernel events
                 M.spmv(dsx, ydev, kernel, kind, assign=assign)
         fl.synchronize()
         prep_s = time.time() - t0
@@ -307,11 +315,14 @@ def run_batched(fl, count, seed, steps, warmup, sequential=True, ks=(1, 8, 32)):
     B = fl.FastLoadMatrix(M_, N_, cp, ri, v, width=64)
     B.prepare_rows()
     B1 = B if sched.width == 64 else fl.FastLoadMatrix(M_, N_, cp, ri, v, width=sched.width)
-    batched_sched = sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB)
-    if sched.kernel >= 0:
-        B1.prepare(fl.VAL_F32)
-    if batched_sched:
-        B1.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
+    def prep_sched(Mx):
+        if sched.kernel >= 0:
+            Mx.prepare(fl.VAL_F32)
+        if sched.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
+            Mx.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
+        elif sched.kernel == fl.KERNEL_HYB:
+            Mx.prepare_hyb(None, sched.batch_tiles, sched.band_rows)
+    prep_sched(B1)
     res = {"matrices": count, "rows": int(M_), "nnz": nnz, "learned_schedule_k1": chosen}
 
     def timed(fn, reps, warm=None):
@@ -335,10 +346,7 @@ def run_batched(fl, count, seed, steps, warmup, sequential=True, ks=(1, 8, 32)):
         Q = fl.FastLoadMatrix(m, n, c, r, vv, width=64)
         Q.prepare_rows()
         S = Q if sched.width == 64 else fl.FastLoadMatrix(m, n, c, r, vv, width=sched.width)
-        if sched.kernel >= 0:
-            S.prepare(fl.VAL_F32)
-        if batched_sched:
-            S.prepare_schedule(sched.kernel, sched.batch_tiles, max(sched.band_rows, 1024))
+        prep_sched(S)
         singles.append((P, Q, S, P.perm(), S.perm() if sched.kernel >= 0 else None))
     log(f"[batched] per-matrix preprocessing done ({time.time() - t0:.1f}s)")
     for k in ks:
@@ -705,7 +713,7 @@ def main():
     ap.add_argument("--batch-k", default="1,8,32", help="right-hand-side counts of the batched configuration")
     ap.add_argument("--no-seq", action="store_true", help="skip the batched configuration's sequential baselines")
     ap.add_argument("--no-powerlaw", action="store_true", help="skip the power-law workload (profiling runs)")
-    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32")
+    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32,hyb_f32")
     ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,winr64_f32,pb_f32,row_f32")
     ap.add_argument("--schedule", default=None, help="use this schedule for the learned variants instead of "
                     "selecting one (profiling runs)")

@@ -59,7 +59,11 @@ struct FileHeader {
     int32_t pb_R, pb_nbands, pb_chunks, reserved0;
     int64_t pb_ring, pb_nruns, win_D;
     uint32_t nsections, reserved1;
-    int64_t reserved[8];
+    // HYB schedule: hyb_H1 = heavy columns + 1 when prepared (0: not), with its PB parameters. The
+    // schedule is rebuilt on load from the layout (its sorts are redone, about 40 ms at 67 M
+    // entries) rather than stored.
+    int64_t hyb_H1, hyb_bt, hyb_R;
+    int64_t reserved[5];
 };
 static_assert(sizeof(FileHeader) == 192, "header layout");
 
@@ -430,6 +434,11 @@ int fl_matrix_save(const fl_matrix *A, const char *path, void *stream) {
         w.add(PB_CHUNK_BATCH, A->pb_chunk_batch.data(), C + 1);
         w.add(PB_CHUNK_OFF, A->pb_chunk_off.data(), C);
     }
+    if (A->hyb_ready) {
+        h.hyb_H1 = (int64_t)A->hyb_H + 1;
+        h.hyb_bt = A->hyb_bt;
+        h.hyb_R = A->hyb_R;
+    }
     if (A->row_ready) {
         h.row_ready = 1;
         rptr = to_host(A->row_ptr, m + 1, s);
@@ -565,6 +574,13 @@ int fl_matrix_load(fl_matrix **out, const char *path, void *stream) {
         if (rc) return rc;
     }
     if (h.value_kinds & (1u << FL_VAL_PACKED)) prepare_values(A.get(), FL_VAL_PACKED, s);  // after the schedules
+    if (h.hyb_H1 != 0) {
+        const int64_t H = h.hyb_H1 - 1;
+        if (H < 0 || H > std::min<int64_t>(n, kHybMaxH32) || (h.hyb_bt != 64 && h.hyb_bt != 128 && h.hyb_bt != 256) ||
+            h.hyb_R < 1024 || h.hyb_R > 16384)
+            return r.bad("HYB schedule parameters out of range");
+        hyb_prepare(A.get(), (int)H, (int)h.hyb_bt, (int)h.hyb_R, s);
+    }
     FL_HIP(hipStreamSynchronize(s));
     *out = A.release();
     return FL_OK;

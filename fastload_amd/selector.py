@@ -22,7 +22,9 @@ synthetic population (`population_spec`), with the matrix and x resident on the 
 y = A x in FP32 (`fl_spmv_assign`, or `fl_spmm` for `row`; value storage FP32, FP64 LDS sums
 where the schedule has them). Preprocessing time is not part of a step (it is paid once per
 matrix). On a new matrix the network's probabilities pick the candidates that are timed before
-the fastest is kept (`Selector.choose_refined`, `tuning_candidates`).
+the fastest is kept (`Selector.choose_refined`, `tuning_candidates`). The HYB schedule (heaviest
+columns from LDS, the rest PB) is not one of the network's classes: it is added to the timed
+candidates by a rule, when the 32768 longest columns hold at least half of the entries.
 """
 from __future__ import annotations
 
@@ -37,6 +39,7 @@ import numpy as np
 
 from ._lib import check, lib
 from .api import (GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF, GEN_ZIPF_DENSE, KERNEL_DROPIN, KERNEL_FAST,
+                  KERNEL_HYB,
                   KERNEL_PB, KERNEL_WIN, KERNEL_WINR, VAL_F32, FastLoadMatrix, gen_matrix)
 from .device import DeviceArray, Event, synchronize
 
@@ -80,6 +83,25 @@ FL_SCHEDULES = 12
 SCHEDULES = _schedule_table()
 N_CLASSES = len(SCHEDULES)
 DEFAULT_CLASS = 0
+# Schedules outside the network's classes, timed by choose_refined when a matrix qualifies: the
+# HYB schedule (heaviest HYB_COLS columns from LDS, the rest PB 128 x 16K) when those columns hold
+# at least HYB_MIN_SHARE of the entries. Index N_CLASSES + i in ALL_SCHEDULES.
+HYB_COLS = 32768
+HYB_MIN_SHARE = 0.5
+EXTRA_SCHEDULES = (Schedule("hyb32k_pb128_r16k", 64, KERNEL_HYB, 128, 16384),)
+ALL_SCHEDULES = SCHEDULES + EXTRA_SCHEDULES
+HYB_CLASS = N_CLASSES
+
+
+def heavy_share(n, dcolptr, cols: int = HYB_COLS) -> float:
+    """Share of the entries held by the `cols` longest columns of a device CSC matrix."""
+    cp = dcolptr.to_host()[:n + 1].astype(np.int64)
+    nnz = int(cp[n]) if n else 0
+    if nnz == 0:
+        return 0.0
+    cnt = np.diff(cp)
+    k = min(cols, n)
+    return float(np.partition(cnt, n - k)[n - k:].sum()) / nnz
 
 DATA_DIR = Path(__file__).resolve().parent / "data"
 DATASET_PATH = DATA_DIR / "selector_sweep.npz"
@@ -304,14 +326,17 @@ class Selector:
         """Model-guided tuning: the network ranks the schedules and the candidates are timed on
         this matrix (`reps` steps each, after warm-up steps); the fastest is returned with the
         measured times. Candidates: the `top` best-ranked, or by default (top=None) those of
-        tuning_candidates (probability >= TUNE_MIN_PROB, at most TUNE_MAX). One-time cost:
-        preparing and timing the candidates."""
+        tuning_candidates (probability >= TUNE_MIN_PROB, at most TUNE_MAX), plus the HYB schedule
+        (HYB_CLASS, an index of ALL_SCHEDULES) when the HYB_COLS longest columns hold at least
+        HYB_MIN_SHARE of the entries. One-time cost: preparing and timing the candidates."""
         f = features(m, n, dcolptr, drowidx)
         if top is None:
             p = self.probs_features(f)
             cands = [DEFAULT_CLASS] if p is None else tuning_candidates(p)
         else:
             cands = self.rank_features(f)[:max(top, 1)]
+        if heavy_share(n, dcolptr) >= HYB_MIN_SHARE:
+            cands = cands + [HYB_CLASS]
         if len(cands) == 1:
             return cands[0], {cands[0]: None}, f
         mats, times = {}, {}
@@ -334,7 +359,7 @@ class ScheduledSpMV:
 
     def __init__(self, m, n, dcolptr, drowidx, dval, cls: int, mats=None):
         self.cls = cls
-        self.sched = s = SCHEDULES[cls]
+        self.sched = s = ALL_SCHEDULES[cls]
         self.m, self.n = m, n
         key = s.width
         if mats is not None and key in mats:
@@ -350,6 +375,8 @@ class ScheduledSpMV:
             M.prepare(VAL_F32)
             if s.kernel in (KERNEL_WIN, KERNEL_WINR, KERNEL_PB):
                 M.prepare_schedule(s.kernel, s.batch_tiles, max(s.band_rows, 1024))
+            elif s.kernel == KERNEL_HYB:
+                M.prepare_hyb(min(n, HYB_COLS), s.batch_tiles, s.band_rows)
         self.dsx = None
 
     def set_x(self, dx: DeviceArray):

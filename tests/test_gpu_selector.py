@@ -194,7 +194,11 @@ def test_refined_choice_times_the_top_ranked(gpu):
     c1, _, _ = s.choose(m, n, dcp, dri)
     assert ranked[0] == c1
     c, times, f = s.choose_refined(m, n, dcp, dri, dv, top=3)
-    assert set(times) == set(ranked[:3]) and c == min(times, key=times.get)
+    # the HYB schedule joins the timed candidates when the 32768 longest columns hold half of
+    # the entries (they do on this Zipf matrix)
+    share = sel.heavy_share(n, dcp)
+    assert share >= sel.HYB_MIN_SHARE
+    assert set(times) == set(ranked[:3]) | {sel.HYB_CLASS} and c == min(times, key=times.get)
     assert all(t > 0 for t in times.values())
     run = sel.ScheduledSpMV(m, n, dcp, dri, dv, c)
     x = np.random.default_rng(2).standard_normal(n).astype(np.float32)
@@ -203,6 +207,19 @@ def test_refined_choice_times_the_top_ranked(gpu):
     run.step(dy)
     yo, S = ob.spmv_oracle(m, n, cp, ri, v.astype(np.float32), x)
     assert rel_err(dy.to_host().astype(np.float64), yo, S) <= 1e-5
+
+
+def test_heavy_share_decides_the_hyb_candidate(gpu):
+    """heavy_share is the share of entries in the 32768 longest columns; the HYB schedule is timed
+    only above HYB_MIN_SHARE (a 27-point stencil has no heavy columns: 32768 of 110592 equal
+    columns hold 29.6 % of the entries)."""
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_STENCIL27, 15, 0, 0, (48,))
+    dcp, dri, dv = DeviceArray.from_host(cp), DeviceArray.from_host(ri), DeviceArray.from_host(v)
+    cnt = np.sort(np.diff(cp))[::-1]
+    want = cnt[:32768].sum() / cp[n]
+    assert abs(sel.heavy_share(n, dcp) - want) < 1e-12 and want < sel.HYB_MIN_SHARE
+    _, times, _ = sel.Selector().choose_refined(m, n, dcp, dri, dv, top=2)
+    assert sel.HYB_CLASS not in times
 
 
 def test_native_selection_matches_features_and_network(gpu):

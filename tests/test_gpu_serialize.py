@@ -125,3 +125,41 @@ def _loads(tmp_path, raw):
     q.write_bytes(raw)
     L = fl.FastLoadMatrix.load(q)
     return L.nnz > 0
+
+
+def test_save_load_rebuilds_hyb(gpu, tmp_path):
+    """A matrix saved with the HYB schedule prepared records its split and PB parameters; the
+    loaded matrix runs HYB without preparing it again (FP64, FP32 and packed values) and matches
+    the FP64 oracle and the original; out-of-range HYB parameters in the header are refused."""
+    import struct
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, 79, 50000, 60000, (1.8, 6000))
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    M.prepare(fl.VAL_F32)
+    M.prepare_hyb(4096, 128, 8192)
+    M.prepare(fl.VAL_PACKED)
+    path = tmp_path / "hyb.flmx"
+    M.save(path)
+    L = fl.FastLoadMatrix.load(path)
+    assert L.hyb_info() == M.hyb_info()
+    x = np.random.default_rng(9).standard_normal(n)
+    for kind in (fl.VAL_F64, fl.VAL_F32, fl.VAL_PACKED):
+        dt = np.float64 if kind == fl.VAL_F64 else np.float32
+        ys = []
+        for A in (L, M):
+            dsx = DeviceArray(n, dt)
+            A.permute_x(DeviceArray.from_host(x.astype(dt)), dsx)
+            dy = DeviceArray(m, dt)
+            A.spmv(dsx, dy, fl.KERNEL_HYB, kind, assign=True)
+            ys.append(dy.to_host().astype(np.float64))
+        xv = x if kind == fl.VAL_F64 else x.astype(np.float32).astype(np.float64)
+        y_or, sc = ob.spmv_oracle(m, n, cp, ri, v, xv)
+        tol = 1e-12 if kind == fl.VAL_F64 else (1e-5 if kind == fl.VAL_F32 else 2.0 ** -13)
+        assert rel_err(ys[0], y_or, sc) <= tol, kind
+        assert rel_err(ys[0], ys[1], sc) <= tol, kind
+    raw = bytearray(path.read_bytes())
+    assert struct.unpack_from("<qqq", raw, 128) == (4097, 128, 8192)
+    struct.pack_into("<q", raw, 128, n + 2)
+    q = tmp_path / "bad_hyb.flmx"
+    q.write_bytes(bytes(raw))
+    with pytest.raises(fl.FastLoadError, match="HYB schedule parameters"):
+        fl.FastLoadMatrix.load(q)
