@@ -405,7 +405,22 @@ __global__ __launch_bounds__(1024) void k_hyb(int m, int H, int nslice, const lo
                                               const X *__restrict__ xh, Y *__restrict__ ys) {
     extern __shared__ double hyb_lds[];
     X *xs = reinterpret_cast<X *>(hyb_lds);
-    for (int i = threadIdx.x; i <= H; i += 1024) xs[i] = i < H ? xh[i] : X(0);
+    // stage x: 8 loads in flight per thread before their LDS stores (one load-store round trip
+    // per element would keep every wave waiting on L2 latency about 10 us before its first slice)
+    constexpr int kStage = 8;
+    for (int base = 0; base <= H; base += 1024 * kStage) {
+        X r[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int i = base + j * 1024 + (int)threadIdx.x;
+            r[j] = i < H ? xh[i] : X(0);
+        }
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int i = base + j * 1024 + (int)threadIdx.x;
+            if (i <= H) xs[i] = r[j];
+        }
+    }
     __syncthreads();
     const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     const int nw = (int)gridDim.x * 16, g0 = (int)blockIdx.x * 16 + wave;
