@@ -24,7 +24,7 @@ where the schedule has them). Preprocessing time is not part of a step (it is pa
 matrix). On a new matrix the network's probabilities pick the candidates that are timed before
 the fastest is kept (`Selector.choose_refined`, `tuning_candidates`). The HYB schedule (heaviest
 columns from LDS, the rest PB) is not one of the network's classes: it is added to the timed
-candidates by a rule, when the 32768 longest columns hold at least half of the entries.
+candidates by a rule, when the 32768 longest columns hold at least 35 % of the entries.
 """
 from __future__ import annotations
 
@@ -85,9 +85,10 @@ N_CLASSES = len(SCHEDULES)
 DEFAULT_CLASS = 0
 # Schedules outside the network's classes, timed by choose_refined when a matrix qualifies: the
 # HYB schedule (heaviest HYB_COLS columns from LDS, the rest PB 128 x 16K) when those columns hold
-# at least HYB_MIN_SHARE of the entries. Index N_CLASSES + i in ALL_SCHEDULES.
+# at least HYB_MIN_SHARE of the entries (tuning keeps it only when it is the fastest). Index
+# N_CLASSES + i in ALL_SCHEDULES.
 HYB_COLS = 32768
-HYB_MIN_SHARE = 0.5
+HYB_MIN_SHARE = 0.35
 EXTRA_SCHEDULES = (Schedule("hyb32k_pb128_r16k", 64, KERNEL_HYB, 128, 16384),)
 ALL_SCHEDULES = SCHEDULES + EXTRA_SCHEDULES
 HYB_CLASS = N_CLASSES

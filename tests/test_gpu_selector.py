@@ -194,8 +194,8 @@ def test_refined_choice_times_the_top_ranked(gpu):
     c1, _, _ = s.choose(m, n, dcp, dri)
     assert ranked[0] == c1
     c, times, f = s.choose_refined(m, n, dcp, dri, dv, top=3)
-    # the HYB schedule joins the timed candidates when the 32768 longest columns hold half of
-    # the entries (they do on this Zipf matrix)
+    # the HYB schedule joins the timed candidates when the 32768 longest columns hold at least
+    # HYB_MIN_SHARE of the entries (they do on this Zipf matrix)
     share = sel.heavy_share(n, dcp)
     assert share >= sel.HYB_MIN_SHARE
     assert set(times) == set(ranked[:3]) | {sel.HYB_CLASS} and c == min(times, key=times.get)
