@@ -15,9 +15,11 @@ Variants (schedules x value storage):
   winr*    the same kernel on tiles grouped by row window (most rows owned by one batch)
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
   row_*    entries re-ordered by row, four lanes per row, no atomics
+  hyb_*    the min(n, 32768) longest columns row by row with their x staged in LDS, the other
+           columns through PB on a sub-matrix
   learned_* the schedule the learned selector picks for the matrix: features and MLP inference
-           on the GPU (committed trained weights) rank the 12 schedules, those of probability
-           >= 5 % (at most 5) are timed on the matrix and the fastest is used; in each value
+           on the GPU (committed trained weights) rank the 13 schedules, those of probability
+           >= 2 % (at most 5) are timed on the matrix and the fastest is used; in each value
            storage it supports
   *_f32 / *_f64 / *_bf16 / *_bf16x2: value storage (split hi+lo BF16 recombined in FP64)
 `value` = GB/s-equivalent (algorithmic bytes with FP32 values / step time) of the fastest
@@ -139,14 +141,15 @@ def time_variant(fl, run, dy, steps, warmup, zero=True, flush=None):
 
 def select_schedule(fl, m, n, dcp, dri, dv, top=None, force=None):
     """The learned selector on the resident matrix: features and MLP inference on the GPU rank the
-    12 schedules; the schedules the network gives at least 5 % probability (at most 5; or the
-    `top` best-ranked) are timed on the matrix (model-guided tuning) and the fastest is used.
+    13 schedules; the schedules the network gives at least 2 % probability (at most 5; or the
+    `top` best-ranked), and HYB when the 32768 longest columns hold at least 35 % of the entries,
+    are timed on the matrix (model-guided tuning) and the fastest is used.
     Both the network's first choice and the tuned one are reported. `force` (a schedule name)
     skips the selection, for profiling runs whose timings the profiler skews."""
     from fastload_amd import selector as sel
     if force:
-        c = next(i for i, sc in enumerate(sel.ALL_SCHEDULES) if sc.name == force)
-        return sel.ALL_SCHEDULES[c], dict(cls=c, schedule=force, forced=True)
+        c = next(i for i, sc in enumerate(sel.SCHEDULES) if sc.name == force)
+        return sel.SCHEDULES[c], dict(cls=c, schedule=force, forced=True)
     s = sel.Selector()
     fl.synchronize()
     t0 = time.perf_counter()
@@ -155,10 +158,10 @@ def select_schedule(fl, m, n, dcp, dri, dv, top=None, force=None):
     t0 = time.perf_counter()
     c, times, _ = s.choose_refined(m, n, dcp, dri, dv, top=top)
     ms = (time.perf_counter() - t0) * 1e3
-    return sel.ALL_SCHEDULES[c], dict(cls=c, schedule=sel.ALL_SCHEDULES[c].name,
+    return sel.SCHEDULES[c], dict(cls=c, schedule=sel.SCHEDULES[c].name,
                                       network_choice=sel.SCHEDULES[c1].name, network_prob=round(p, 4),
                                       select_ms=round(ms1, 3), tuned_candidates=len(times),
-                                      tuned_ms={sel.ALL_SCHEDULES[k].name: (round(v, 4) if v is not None else None)
+                                      tuned_ms={sel.SCHEDULES[k].name: (round(v, 4) if v is not None else None)
                                                 for k, v in times.items()},
                                       tuning_s=round(ms / 1e3, 3), trained=s.mlp is not None)
 

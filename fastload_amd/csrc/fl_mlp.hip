@@ -363,6 +363,7 @@ static const int kSchedules[FL_SCHEDULES][4] = {
     {64, FL_KERNEL_WIN, 64, 0},     {64, FL_KERNEL_WIN, 128, 0},   {64, FL_KERNEL_WIN, 256, 0},
     {64, FL_KERNEL_PB, 128, 4096},  {64, FL_KERNEL_PB, 128, 16384}, {64, FL_KERNEL_PB, 256, 4096},
     {64, FL_KERNEL_PB, 256, 8192},  {64, FL_KERNEL_PB, 256, 16384}, {64, FL_KERNEL_ROW, 0, 0},
+    {64, FL_KERNEL_HYB, 128, 16384},  // heavy columns min(n, 32768), the rest PB 128 x 16K
 };
 
 int fl_schedule_info(int cls, int *width, int *kernel, int *batch_tiles, int *band_rows) {
@@ -381,7 +382,7 @@ int fl_select_schedule(fl_mlp *P, int m, int n, int nnz, const int *d_colptr, co
     *cls = 0;
     if (prob) *prob = 0.f;
     if (P && (P->d[0] != 40 || P->d[3] != FL_SCHEDULES))
-        return mlp_invalid("fl_select_schedule: network must map 40 features to the 12 schedules");
+        return mlp_invalid("fl_select_schedule: network must map 40 features to the FL_SCHEDULES schedules");
     float f[40];
     const int rc = fl_features(m, n, nnz, d_colptr, d_rowidx, f, stream);
     if (rc != FL_OK) return rc;

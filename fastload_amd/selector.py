@@ -1,6 +1,6 @@
-"""Learned schedule selector: 40 matrix features computed on the GPU -> a 40-64-64-12 perceptron
+"""Learned schedule selector: 40 matrix features computed on the GPU -> a 40-64-64-13 perceptron
 trained and run by the library's own HIP kernels (fl_mlp_*: MFMA with BF16 weights and FP32
-accumulation) -> one of 12 SpMV schedules.
+accumulation) -> one of 13 SpMV schedules.
 
 The reference has one fixed schedule (the 32-wide tiles of src/formatTransform_GPU.h with the
 segmented-sum kernel of src/FastLoad_GPU.h); that schedule is class 0, the selector's default and
@@ -16,15 +16,16 @@ Schedule space (tile width x reduction strategy x batch/band parameters):
                  row of a batch (a store for rows no other batch touches, else one atomic)
   6-10 pb*       propagation blocking, batch 128/256 tiles x band 4K/8K/16K rows, no atomics
   11 row         entries re-ordered by row (CSR), four lanes per row, no atomics (fl_spmm, k=1)
+  12 hyb32k_pb128_r16k  the min(n, 32768) longest columns row by row with their x in LDS, the
+                 rest PB 128 tiles x 16K rows on a sub-matrix
 
 Training data come from exhaustive sweeps: every schedule is timed on every matrix of a seeded
 synthetic population (`population_spec`), with the matrix and x resident on the GPU; one step is
 y = A x in FP32 (`fl_spmv_assign`, or `fl_spmm` for `row`; value storage FP32, FP64 LDS sums
 where the schedule has them). Preprocessing time is not part of a step (it is paid once per
 matrix). On a new matrix the network's probabilities pick the candidates that are timed before
-the fastest is kept (`Selector.choose_refined`, `tuning_candidates`). The HYB schedule (heaviest
-columns from LDS, the rest PB) is not one of the network's classes: it is added to the timed
-candidates by a rule, when the 32768 longest columns hold at least 35 % of the entries.
+the fastest is kept (`Selector.choose_refined`, `tuning_candidates`); the HYB schedule is timed
+as well whenever the 32768 longest columns hold at least 35 % of the entries.
 """
 from __future__ import annotations
 
@@ -38,7 +39,8 @@ from pathlib import Path
 import numpy as np
 
 from ._lib import check, lib
-from .api import (GEN_BANDED, GEN_BLOCKDIAG, GEN_UNIFORM, GEN_ZIPF, GEN_ZIPF_DENSE, KERNEL_DROPIN, KERNEL_FAST,
+from .api import (GEN_BANDED, GEN_BLOCKDIAG, GEN_STENCIL27, GEN_UNIFORM, GEN_ZIPF, GEN_ZIPF_DENSE, KERNEL_DROPIN,
+                  KERNEL_FAST,
                   KERNEL_HYB,
                   KERNEL_PB, KERNEL_WIN, KERNEL_WINR, VAL_F32, FastLoadMatrix, gen_matrix)
 from .device import DeviceArray, Event, synchronize
@@ -57,11 +59,14 @@ class Schedule:
     band_rows: int = 0
 
 
+HYB_COLS = 32768  # heavy columns of the HYB class: min(n, HYB_COLS)
+
+
 def _schedule_table():
-    """The 12 schedules, read from the library (fl_schedule_info) so C and Python callers share
+    """The 13 schedules, read from the library (fl_schedule_info) so C and Python callers share
     one table; names follow the parameters (dropin_w32, win128, pb256_r16k, row, ...)."""
     kname = {KERNEL_DROPIN: "dropin", KERNEL_FAST: "fast", KERNEL_WIN: "win", KERNEL_WINR: "winr", KERNEL_PB: "pb",
-             KERNEL_ROW: "row"}
+             KERNEL_ROW: "row", KERNEL_HYB: "hyb"}
     out = []
     for c in range(FL_SCHEDULES):
         w, k, bt, r = C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -73,25 +78,22 @@ def _schedule_table():
             name = f"{kname[k]}{bt}"
         elif k == KERNEL_PB:
             name = f"pb{bt}_r{r // 1024}k"
+        elif k == KERNEL_HYB:
+            name = f"hyb{HYB_COLS // 1024}k_pb{bt}_r{r // 1024}k"
         else:
             name = "row"
         out.append(Schedule(name, w, k, bt, r))
     return tuple(out)
 
 
-FL_SCHEDULES = 12
+FL_SCHEDULES = 13
 SCHEDULES = _schedule_table()
 N_CLASSES = len(SCHEDULES)
 DEFAULT_CLASS = 0
-# Schedules outside the network's classes, timed by choose_refined when a matrix qualifies: the
-# HYB schedule (heaviest HYB_COLS columns from LDS, the rest PB 128 x 16K) when those columns hold
-# at least HYB_MIN_SHARE of the entries (tuning keeps it only when it is the fastest). Index
-# N_CLASSES + i in ALL_SCHEDULES.
-HYB_COLS = 32768
+HYB_CLASS = next(c for c, sc in enumerate(SCHEDULES) if sc.kernel == KERNEL_HYB)
+# choose_refined also times the HYB class when the HYB_COLS longest columns hold at least
+# HYB_MIN_SHARE of the entries, whatever probability the network gives it
 HYB_MIN_SHARE = 0.35
-EXTRA_SCHEDULES = (Schedule("hyb32k_pb128_r16k", 64, KERNEL_HYB, 128, 16384),)
-ALL_SCHEDULES = SCHEDULES + EXTRA_SCHEDULES
-HYB_CLASS = N_CLASSES
 
 
 def heavy_share(n, dcolptr, cols: int = HYB_COLS) -> float:
@@ -107,8 +109,9 @@ def heavy_share(n, dcolptr, cols: int = HYB_COLS) -> float:
 DATA_DIR = Path(__file__).resolve().parent / "data"
 DATASET_PATH = DATA_DIR / "selector_sweep.npz"
 # Model-guided tuning times every schedule the network gives at least TUNE_MIN_PROB probability
-# (the TUNE_MAX most likely at most, the top one always).
-TUNE_MIN_PROB = 0.05
+# (the TUNE_MAX most likely at most, the top one always). On the training matrices 2 % keeps the
+# regret of timing the top three (1.0001 against 1.0005) with two timings on average.
+TUNE_MIN_PROB = 0.02
 TUNE_MAX = 5
 WEIGHTS_PATH = DATA_DIR / "selector_weights.npz"
 
@@ -125,7 +128,7 @@ def features(m: int, n: int, dcolptr: DeviceArray, drowidx: DeviceArray, stream=
 
 # ------------------------------------------------------------------------------------ MLP
 class MLP:
-    """The 40-64-64-12 perceptron of fl_mlp_* (forward, backward and Adam on the GPU)."""
+    """The 40-64-64-13 perceptron of fl_mlp_* (forward, backward and Adam on the GPU)."""
 
     def __init__(self, dims=(N_FEATURES, *HIDDEN, N_CLASSES), seed: int = 0):
         h = C.c_void_p()
@@ -218,9 +221,15 @@ def soft_targets(times: np.ndarray, tau: float = 0.05) -> np.ndarray:
 def train(F, times, idx, steps=3000, batch=128, lr=3e-3, tau=0.05, seed=0, log_every=0):
     """Trains a fresh MLP on rows `idx` of (F, times) with Adam on the GPU. Each epoch uploads a
     seeded permutation of the training rows once; minibatches are consecutive slices of it.
-    Returns (mlp, per-step loss)."""
-    F = np.asarray(F, np.float32)[idx]
-    T = soft_targets(np.asarray(times)[idx], tau)
+    Returns (mlp, per-step loss). F is (rows, N_FEATURES), times (rows, N_CLASSES): the training
+    kernels read minibatches of exactly those widths."""
+    F = np.asarray(F, np.float32)
+    times = np.asarray(times)
+    if F.ndim != 2 or F.shape[1] != N_FEATURES or times.ndim != 2 or times.shape != (F.shape[0], N_CLASSES):
+        raise ValueError(f"train: features must be (rows, {N_FEATURES}) and times (rows, {N_CLASSES}); "
+                         f"got {F.shape} and {times.shape}")
+    F = F[idx]
+    T = soft_targets(times[idx], tau)
     mlp = MLP(seed=seed)
     rng = np.random.default_rng(seed)
     N = len(idx)
@@ -327,16 +336,16 @@ class Selector:
         """Model-guided tuning: the network ranks the schedules and the candidates are timed on
         this matrix (`reps` steps each, after warm-up steps); the fastest is returned with the
         measured times. Candidates: the `top` best-ranked, or by default (top=None) those of
-        tuning_candidates (probability >= TUNE_MIN_PROB, at most TUNE_MAX), plus the HYB schedule
-        (HYB_CLASS, an index of ALL_SCHEDULES) when the HYB_COLS longest columns hold at least
-        HYB_MIN_SHARE of the entries. One-time cost: preparing and timing the candidates."""
+        tuning_candidates (probability >= TUNE_MIN_PROB, at most TUNE_MAX), plus the HYB class
+        when the HYB_COLS longest columns hold at least HYB_MIN_SHARE of the entries. One-time
+        cost: preparing and timing the candidates."""
         f = features(m, n, dcolptr, drowidx)
         if top is None:
             p = self.probs_features(f)
             cands = [DEFAULT_CLASS] if p is None else tuning_candidates(p)
         else:
             cands = self.rank_features(f)[:max(top, 1)]
-        if heavy_share(n, dcolptr) >= HYB_MIN_SHARE:
+        if HYB_CLASS not in cands and heavy_share(n, dcolptr) >= HYB_MIN_SHARE:
             cands = cands + [HYB_CLASS]
         if len(cands) == 1:
             return cands[0], {cands[0]: None}, f
@@ -360,7 +369,7 @@ class ScheduledSpMV:
 
     def __init__(self, m, n, dcolptr, drowidx, dval, cls: int, mats=None):
         self.cls = cls
-        self.sched = s = ALL_SCHEDULES[cls]
+        self.sched = s = SCHEDULES[cls]
         self.m, self.n = m, n
         key = s.width
         if mats is not None and key in mats:
@@ -398,7 +407,7 @@ class ScheduledSpMV:
 
 
 # ------------------------------------------------------------------ sweep population
-POPULATION_FAMILIES = ("zipf", "banded", "blockdiag", "uniform", "zipf_dense")
+POPULATION_FAMILIES = ("zipf", "banded", "blockdiag", "uniform", "zipf_dense", "stencil")
 NNZ_CAP = 72_000_000
 N_MIN, N_MAX = 10_000, 8_388_608
 
@@ -412,17 +421,20 @@ def _zipf_mean(a, kmax):
 def population_spec(i: int, base_seed: int = 20240):
     """Matrix i of the selector population: (family name, gen family, m, n, params, seed).
 
-    Family i % 5; columns log-uniform in [10k, 8M], square; family parameters:
+    Family i % 6; columns log-uniform in [10k, 8M], square; family parameters:
       zipf        exponent U(1.5, 2.5), max column length min(m, 50,000)
       banded      width log-uniform in [5, 200]
       blockdiag   block size log-uniform in [32, 1024], entries per column log-uniform in
                   [2, block]
       uniform     density log-uniform in [1e-6, 1e-3]
       zipf_dense  zipf as above plus 0.1% columns of log-uniform length in [5k, 200k] (<= m)
-    Columns (and rows with them) are then reduced, if needed, so the expected non-zero count stays
-    below 72M, which keeps a sweep bounded while covering the 67M-entry power-law workload."""
+      stencil     27-point stencil on the cube of the nearest side (27 diagonals: multi-band)
+    The expected non-zero count is kept below 72M, which bounds a sweep while covering the
+    67M-entry power-law workload: columns are reduced (not below 10k) for zipf, banded, blockdiag
+    and the stencil; for uniform the density is reduced instead, so every matrix keeps between
+    10k and 8M columns."""
     rng = np.random.default_rng([base_seed, i])
-    fam = POPULATION_FAMILIES[i % 5]
+    fam = POPULATION_FAMILIES[i % len(POPULATION_FAMILIES)]
     n = int(round(math.exp(rng.uniform(math.log(N_MIN), math.log(N_MAX)))))
     seed = base_seed * 100_000 + i
     if fam in ("zipf", "zipf_dense"):
@@ -432,29 +444,33 @@ def population_spec(i: int, base_seed: int = 20240):
         mean = _zipf_mean(a, min(kmax, n))
         if fam == "zipf_dense":
             mean = 0.999 * mean + 0.001 * min(dense_len, n)
-        n = max(N_MIN // 10, min(n, int(NNZ_CAP / mean)))
+        n = max(N_MIN, min(n, int(NNZ_CAP / mean)))
         if fam == "zipf":
             return fam, GEN_ZIPF, n, n, (a, kmax), seed
         return fam, GEN_ZIPF_DENSE, n, n, (a, kmax, 0.001, dense_len), seed
     if fam == "banded":
         w = int(round(math.exp(rng.uniform(math.log(5), math.log(200)))))
-        n = min(n, NNZ_CAP // (w + 1))
+        n = max(N_MIN, min(n, NNZ_CAP // (w + 1)))
         return fam, GEN_BANDED, n, n, (w,), seed
     if fam == "blockdiag":
         b = int(round(math.exp(rng.uniform(math.log(32), math.log(1024)))))
         k = int(round(math.exp(rng.uniform(math.log(2), math.log(b)))))
-        n = min(n, NNZ_CAP // k)
+        n = max(N_MIN, min(n, NNZ_CAP // k))
         return fam, GEN_BLOCKDIAG, n, n, (b, k), seed
+    if fam == "stencil":
+        side = int(round(n ** (1.0 / 3.0)))
+        side = max(22, min(side, int((NNZ_CAP / 27) ** (1.0 / 3.0))))  # 22^3 = 10,648 columns
+        return fam, GEN_STENCIL27, side ** 3, side ** 3, (side,), seed
     d = float(math.exp(rng.uniform(math.log(1e-6), math.log(1e-3))))
-    n = min(n, int(math.sqrt(NNZ_CAP / d)))
+    d = min(d, NNZ_CAP / float(n) / float(n))
     return fam, GEN_UNIFORM, n, n, (d,), seed
 
 
 def population_split(count: int = 2000):
-    """Held-out set: every fifth group of five consecutive matrices (400 of 2,000, 80 per
-    family); the rest train."""
+    """Held-out set: every fifth group of six consecutive matrices (one of each family: 396 of
+    2,000, 66 per family); the rest train."""
     idx = np.arange(count)
-    test = (idx // 5) % 5 == 4
+    test = (idx // 6) % 5 == 4
     return idx[~test], idx[test]
 
 
@@ -588,9 +604,9 @@ def main(argv=None):
     rep = dict(train=len(tr), test=len(te))
     tau, steps = a.tau, a.steps
     if a.tune:
-        # validation: every fifth group of five among the training matrices (never the test set)
-        val = tr[(ids[tr] // 5) % 5 == 3]
-        fit = tr[(ids[tr] // 5) % 5 != 3]
+        # validation: every fifth group of six among the training matrices (never the test set)
+        val = tr[(ids[tr] // 6) % 5 == 3]
+        fit = tr[(ids[tr] // 6) % 5 != 3]
         grid = []
         for g_tau in (0.02, 0.05, 0.1):
             for g_steps in (2000, 5000):

@@ -390,10 +390,11 @@ int fl_mlp_gradients(fl_mlp *P, const float *d_X, const float *d_T, int batch, f
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8, learning rate lr); loss before the step */
 int fl_mlp_train_step(fl_mlp *P, const float *d_X, const float *d_T, int batch, float lr, float *loss, void *stream);
 
-/* The 12 schedules the selector chooses from: class -> tile width (fl_matrix_create), kernel
+/* The 13 schedules the selector chooses from: class -> tile width (fl_matrix_create), kernel
  * (fl_spmv; FL_KERNEL_ROW = fl_spmm with k = 1 after fl_matrix_prepare_rows), batch tiles and
- * band rows (fl_matrix_prepare_schedule). Class 0 is the reference schedule. */
-#define FL_SCHEDULES 12
+ * band rows (fl_matrix_prepare_schedule; for FL_KERNEL_HYB those of its PB sub-matrix, heavy
+ * columns min(n, 32768)). Class 0 is the reference schedule. */
+#define FL_SCHEDULES 13
 int fl_schedule_info(int cls, int *width, int *kernel, int *batch_tiles, int *band_rows);
 /* fl_features + fl_mlp_predict on a device CSC matrix: the network's class and its probability.
  * Falls back to class 0 (probability 0) when net is NULL, the matrix has no non-zeros or a

@@ -112,13 +112,14 @@ def test_fp64_backward_matches_finite_differences():
 
 # ----------------------------------------------------------------------------- host logic
 def test_schedule_table():
-    assert sel.N_CLASSES == 12 and len({s.name for s in sel.SCHEDULES}) == 12
+    assert sel.N_CLASSES == 13 and len({s.name for s in sel.SCHEDULES}) == 13
+    assert sel.SCHEDULES[sel.HYB_CLASS].kernel == fl.KERNEL_HYB
     d = sel.SCHEDULES[sel.DEFAULT_CLASS]
     assert (d.width, d.kernel) == (32, fl.KERNEL_DROPIN)  # the reference schedule
     for s in sel.SCHEDULES:
         if s.kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
             assert s.batch_tiles in (64, 128, 256) and s.width == 64
-        if s.kernel == fl.KERNEL_PB:
+        if s.kernel in (fl.KERNEL_PB, fl.KERNEL_HYB):
             assert 1024 <= s.band_rows <= 16384
 
 
@@ -126,9 +127,11 @@ def test_population_and_split():
     specs = [sel.population_spec(i) for i in range(2000)]
     assert specs[17] == sel.population_spec(17)
     fams = [s[0] for s in specs]
-    assert {f: fams.count(f) for f in set(fams)} == {f: 400 for f in sel.POPULATION_FAMILIES}
+    nf = len(sel.POPULATION_FAMILIES)
+    assert {f: fams.count(f) for f in set(fams)} == {f: 2000 // nf + (k < 2000 % nf)
+                                                     for k, f in enumerate(sel.POPULATION_FAMILIES)}
     for fam, g, m, n, p, seed in specs:
-        assert m == n and 1000 <= n <= sel.N_MAX
+        assert m == n and sel.N_MIN <= n <= sel.N_MAX
         if fam == "zipf":
             exp_nnz = n * sel._zipf_mean(p[0], min(p[1], n))
         elif fam == "zipf_dense":
@@ -138,13 +141,16 @@ def test_population_and_split():
         elif fam == "blockdiag":
             exp_nnz = n * p[1]
             assert 32 <= p[0] <= 1024 and 2 <= p[1] <= p[0]
+        elif fam == "stencil":
+            assert n == p[0] ** 3 and g == fl.GEN_STENCIL27
+            exp_nnz = 27 * n
         else:
             exp_nnz = n * n * p[0]
             assert 1e-6 <= p[0] <= 1e-3
         assert exp_nnz <= sel.NNZ_CAP * 1.001
     tr, te = sel.population_split(2000)
-    assert len(tr) == 1600 and len(te) == 400 and not set(tr) & set(te)
-    assert all(sum(1 for i in te if fams[i] == f) == 80 for f in sel.POPULATION_FAMILIES)
+    assert len(tr) == 1604 and len(te) == 396 and not set(tr) & set(te)
+    assert all(sum(1 for i in te if fams[i] == f) == 66 for f in sel.POPULATION_FAMILIES)
 
 
 def test_soft_targets_and_regret():
@@ -214,7 +220,7 @@ def test_trained_network_gains_over_default_and_fixed_schedule():
     pick = top3[np.arange(len(te)), np.argmin(np.take_along_axis(T[te], top3, axis=1), axis=1)]
     tuned = sel.evaluate(pick, T[te])
     assert tuned["geomean_regret"] <= learned["geomean_regret"] and tuned["exact_best"] >= learned["exact_best"]
-    # the default tuning rule (every schedule of probability >= 5 %, at most 5) is at least as
+    # the default tuning rule (every schedule of probability >= 2 %, at most 5) is at least as
     # good as the top three and times no more schedules than they do on average
     cands = [sel.tuning_candidates(p) for p in sel.softmax(logits)]
     assert all(1 <= len(c) <= sel.TUNE_MAX and c[0] == int(np.argmax(lg)) for c, lg in zip(cands, logits))
@@ -224,12 +230,25 @@ def test_trained_network_gains_over_default_and_fixed_schedule():
     assert np.mean([len(c) for c in cands]) <= 3.0
 
 
+def test_train_rejects_mismatched_widths():
+    """The GPU training kernels read minibatches N_FEATURES and N_CLASSES wide: other widths are
+    refused before anything reaches the device."""
+    F = np.zeros((8, sel.N_FEATURES), np.float32)
+    with pytest.raises(ValueError):
+        sel.train(F, np.ones((8, sel.N_CLASSES - 1)), np.arange(8))
+    with pytest.raises(ValueError):
+        sel.train(F[:, :-1], np.ones((8, sel.N_CLASSES)), np.arange(8))
+    with pytest.raises(ValueError):
+        sel.train(F, np.ones((7, sel.N_CLASSES)), np.arange(7))
+
+
 def test_tuning_candidates_rule():
     """Ordered by probability; the first always, the rest only at >= TUNE_MIN_PROB; at most
     TUNE_MAX."""
     p = np.array([0.01, 0.5, 0.049, 0.2, 0.05, 0.06, 0.07, 0.051, 0.0, 0.0, 0.0, 0.0])
-    assert sel.tuning_candidates(p) == [1, 3, 6, 5, 7]
-    assert sel.tuning_candidates(p, max_top=2) == [1, 3]
+    assert sel.tuning_candidates(p, min_prob=0.05) == [1, 3, 6, 5, 7]
+    assert sel.tuning_candidates(p, min_prob=0.05, max_top=2) == [1, 3]
+    assert sel.tuning_candidates(p) == [1, 3, 6, 5, 7] and sel.tuning_candidates(p, max_top=6) == [1, 3, 6, 5, 7, 4]
     assert sel.tuning_candidates(np.full(12, 1 / 12), min_prob=0.5) == [0]
     q = sel.softmax(np.array([[1.0, 2.0, 3.0]]))
     assert np.allclose(q.sum(), 1.0) and np.argmax(q) == 2
@@ -257,8 +276,8 @@ def test_schedule_table_comes_from_the_library():
     import ctypes as C
     names = [s.name for s in sel.SCHEDULES]
     assert names == ["dropin_w32", "dropin_w64", "winr64", "win64", "win128", "win256", "pb128_r4k", "pb128_r16k",
-                     "pb256_r4k", "pb256_r8k", "pb256_r16k", "row"]
+                     "pb256_r4k", "pb256_r8k", "pb256_r16k", "row", "hyb32k_pb128_r16k"]
     w = C.c_int()
-    assert fl.lib.fl_schedule_info(12, C.byref(w), None, None, None) == -8
+    assert fl.lib.fl_schedule_info(13, C.byref(w), None, None, None) == -8
     assert fl.lib.fl_schedule_info(-1, None, None, None, None) == -8
     assert fl.lib.fl_select_schedule(None, 5, 5, 0, None, None, None, None, None) == -8
