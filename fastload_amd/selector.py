@@ -109,8 +109,8 @@ def heavy_share(n, dcolptr, cols: int = HYB_COLS) -> float:
 DATA_DIR = Path(__file__).resolve().parent / "data"
 DATASET_PATH = DATA_DIR / "selector_sweep.npz"
 # Model-guided tuning times every schedule the network gives at least TUNE_MIN_PROB probability
-# (the TUNE_MAX most likely at most, the top one always). On the training matrices 2 % keeps the
-# regret of timing the top three (1.0001 against 1.0005) with two timings on average.
+# (the TUNE_MAX most likely at most, the top one always). On the training matrices 2 % gives a
+# lower regret than timing the top three (1.0001 against 1.0008) with fewer timings (2.6).
 TUNE_MIN_PROB = 0.02
 TUNE_MAX = 5
 WEIGHTS_PATH = DATA_DIR / "selector_weights.npz"
@@ -467,10 +467,10 @@ def population_spec(i: int, base_seed: int = 20240):
 
 
 def population_split(count: int = 2000):
-    """Held-out set: every fifth group of six consecutive matrices (one of each family: 396 of
-    2,000, 66 per family); the rest train."""
+    """Held-out set: every fifth group of five consecutive matrices (400 of 2,000, 66 or 67 of
+    each of the six families); the rest train."""
     idx = np.arange(count)
-    test = (idx // 6) % 5 == 4
+    test = (idx // 5) % 5 == 4
     return idx[~test], idx[test]
 
 
@@ -604,9 +604,9 @@ def main(argv=None):
     rep = dict(train=len(tr), test=len(te))
     tau, steps = a.tau, a.steps
     if a.tune:
-        # validation: every fifth group of six among the training matrices (never the test set)
-        val = tr[(ids[tr] // 6) % 5 == 3]
-        fit = tr[(ids[tr] // 6) % 5 != 3]
+        # validation: every fifth group of five among the training matrices (never the test set)
+        val = tr[(ids[tr] // 5) % 5 == 3]
+        fit = tr[(ids[tr] // 5) % 5 != 3]
         grid = []
         for g_tau in (0.02, 0.05, 0.1):
             for g_steps in (2000, 5000):

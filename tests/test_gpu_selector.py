@@ -136,7 +136,7 @@ needs_data = pytest.mark.skipif(not (sel.DATASET_PATH.exists() and sel.WEIGHTS_P
 
 @needs_data
 def test_selector_gain_on_held_out_matrices(gpu):
-    """The committed network run by the GPU kernels on the 396 held-out sweep matrices: it beats
+    """The committed network run by the GPU kernels on the 400 held-out sweep matrices: it beats
     the default schedule and the best fixed schedule; a network retrained here from the sweep
     with the GPU training kernels does too."""
     ids, F, T, E, M = sel.load_dataset([sel.DATASET_PATH])
