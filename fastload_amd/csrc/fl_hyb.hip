@@ -18,7 +18,10 @@
 //     power-law workload the padding adds 5 % to the heavy entries. A wave reads one slot per
 //     lane per pair row: 8 (16 in FP64) + 4 contiguous bytes, with no per-entry index math;
 //   - the heavy kernel writes y by sorted position (hyb_ys), and the sub-matrix's first PB phase 2
-//     adds hyb_ys[(r & ~(kHybWin - 1)) + hyb_iperm[r]] to row r while it writes y.
+//     adds it to y: with bands of kHybWin rows (one band = one window) it reads hyb_ys in order
+//     and adds hyb_ys[p] to row hyb_perm[p] of its LDS sums (a gather through hyb_iperm would
+//     move a 128-byte line per 4-byte value: 14 of phase 2's 42 us on the power-law workload);
+//     with other band sizes it adds hyb_ys[(r & ~(kHybWin - 1)) + hyb_iperm[r]] to row r.
 // The column sort is ascending, so the heavy columns are the last H sorted columns: heavy x is
 // sortx + (n - H), and the sub-matrix's sorted columns are the first n - H sorted columns in the
 // same order (its own column sort of already ascending counts is the identity, so it is not run),
@@ -105,10 +108,12 @@ __global__ void k_hyb_winkey(const int *__restrict__ rows, int m, uint32_t *__re
     HSTRIDE(p, m) key[p] = (uint32_t)(rows[p] >> kHybWinLog);
 }
 
-__global__ void k_hyb_pos(const int *__restrict__ rows, int m, uint16_t *__restrict__ iperm, int *__restrict__ flag) {
+__global__ void k_hyb_pos(const int *__restrict__ rows, int m, uint16_t *__restrict__ iperm,
+                          uint16_t *__restrict__ perm, int *__restrict__ flag) {
     HSTRIDE(p, m) {
         const int r = rows[p];
         iperm[r] = (uint16_t)(p & (kHybWin - 1));
+        perm[p] = (uint16_t)(r & (kHybWin - 1));
         if ((p >> kHybWinLog) != (r >> kHybWinLog)) *flag = 1;  // a row left its window
     }
 }
@@ -247,6 +252,7 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
     A->hyb_sb.alloc((size_t)nslice + 1);
     A->hyb_np.alloc(std::max(nslice, 1));
     A->hyb_iperm.alloc(std::max(m, 1));
+    A->hyb_perm.alloc(std::max(m, 1));
     long long slots = 0;
     DevBuf<int> hptr((size_t)m + 1), rows(std::max(m, 1)), flag(1);
     DevBuf<uint32_t> rkey(std::max(nnz_h, 1));
@@ -269,7 +275,7 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
         k_hyb_winkey<<<hg(m), kBlock, 0, s>>>(rows.p, m, wkey.p);
         FL_LAUNCH_CHECK();
         radix_sort_pairs(wkey.p, rows.p, m, bits_for((uint32_t)((m - 1) >> kHybWinLog)), s);
-        k_hyb_pos<<<hg(m), kBlock, 0, s>>>(rows.p, m, A->hyb_iperm.p, flag.p);
+        k_hyb_pos<<<hg(m), kBlock, 0, s>>>(rows.p, m, A->hyb_iperm.p, A->hyb_perm.p, flag.p);
         DevBuf<int> pairs(std::max(nslice, 1)), pscan((size_t)nslice + 1);
         k_hyb_pairs<<<hg(nslice), kBlock, 0, s>>>(rows.p, hptr.p, m, nslice, A->hyb_np.p, pairs.p);
         FL_LAUNCH_CHECK();
@@ -529,9 +535,15 @@ int spmv_hyb(fl_matrix *A, int kind, const void *x, void *y, bool assign, hipStr
         if (dbl) launch_hyb<double2, double, double>(A, (const double2 *)A->hyb_val64.p, (const double *)x + nl, (double *)ys, s);
         else launch_hyb<float2, float, float>(A, (const float2 *)A->hyb_val32.p, (const float *)x + nl, (float *)ys, s);
     }
-    if (L->tilenum > 0)
-        return spmv_batched(L, FL_KERNEL_PB, kind, x, y, assign, s, bins, heavy ? ys : nullptr,
-                            heavy ? A->hyb_iperm.p : nullptr);
+    if (L->tilenum > 0) {
+        PbFold fold;
+        if (heavy) {
+            fold.ys = ys;
+            fold.iperm = A->hyb_iperm.p;
+            fold.perm = A->hyb_perm.p;
+        }
+        return spmv_batched(L, FL_KERNEL_PB, kind, x, y, assign, s, bins, &fold);
+    }
     if (heavy) {
         if (dbl) {
             if (assign) k_hyb_unperm<double, true><<<hg(m), kBlock, 0, s>>>(m, A->hyb_iperm.p, (const double *)ys, (double *)y);

@@ -120,6 +120,7 @@ struct fl_matrix {
     fl::DevBuf<long long> hyb_sb;     // [nslice + 1] first slot of every slice
     fl::DevBuf<int> hyb_np;           // [nslice] pair rows of every slice (32-bit: scalar loads)
     fl::DevBuf<uint16_t> hyb_iperm;   // [m] sorted position of each row inside its window
+    fl::DevBuf<uint16_t> hyb_perm;    // [m] row (inside its window) at each sorted position
     fl::DevBuf<uint32_t> hyb_col2;    // [hyb_slots + pad] two columns per slot
     fl::DevBuf<double> hyb_val64;     // [2 (hyb_slots + pad)] two values per slot
     fl::DevBuf<float> hyb_val32;      // built with the FP32 values
@@ -169,10 +170,20 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
 int spmv(fl_matrix *A, int kernel, int kind, const void *x, void *y, void *work, bool assign, hipStream_t s,
          void *pb_bins = nullptr);
 
-// The WIN/WINR/PB launches (fl_sched.hip). fold_ys/fold_iperm (PB only): the first chunk's phase 2
-// also adds fold_ys[(r & ~(kHybWin - 1)) + fold_iperm[r]] to row r (the HYB schedule's heavy part).
+// The HYB schedule's heavy part, which the first chunk's phase 2 of its PB sub-matrix adds to y.
+// ys holds the heavy sum of row r at (r & ~(kHybWin - 1)) + iperm[r] (its position inside its
+// window); perm is the inverse (position -> row inside the window). When the PB bands are the
+// windows (band rows == kHybWin), phase 2 reads ys in order and adds ys[p] to row perm[p] of the
+// band in LDS; otherwise it gathers ys through iperm as it writes each row.
+struct PbFold {
+    const void *ys = nullptr;
+    const uint16_t *iperm = nullptr;
+    const uint16_t *perm = nullptr;
+};
+
+// The WIN/WINR/PB launches (fl_sched.hip); fold: PB only, see PbFold.
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
-                 void *pb_bins, const void *fold_ys = nullptr, const uint16_t *fold_iperm = nullptr);
+                 void *pb_bins, const PbFold *fold = nullptr);
 
 // HYB schedule (fl_hyb.hip). Heavy x is staged in LDS: at most kHybMaxH32 FP32 or kHybMaxH64
 // FP64 columns.

@@ -977,13 +977,16 @@ struct PbVec<double> {
 
 // ASSIGN: y = A x (every row of the band is written, so y needs no clearing); else y += A x.
 // band_start points at one chunk's band table; the product at band-major position i is
-// bins[i - off] (off a multiple of 4), its row rowlocal[i]. FOLD: row r also gets
-// fys[(r & ~(kHybWin - 1)) + fiperm[r]] (a part of y kept in another row order, added in FP64).
-template <typename P, typename Y, int U, bool ASSIGN, bool FOLD>
+// bins[i - off] (off a multiple of 4), its row rowlocal[i]. FOLD adds a part of y kept in another
+// row order (PbFold), in FP64: kFoldGather adds fys[(r & ~(kHybWin - 1)) + fmap[r]] to row r as it
+// is written; kFoldScan (bands of kHybWin rows, so band b is window b) reads the band's fys in
+// order after the products and adds fys[p] to the accumulator of row fmap[p] of the band.
+constexpr int kFoldNone = 0, kFoldGather = 1, kFoldScan = 2;
+template <typename P, typename Y, int U, bool ASSIGN, int FOLD>
 __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
                                                    const P *__restrict__ bins, int off,
                                                    const uint16_t *__restrict__ rowlocal, Y *__restrict__ y,
-                                                   const Y *__restrict__ fys, const uint16_t *__restrict__ fiperm) {
+                                                   const Y *__restrict__ fys, const uint16_t *__restrict__ fmap) {
     extern __shared__ double acc[];
     using T = PbVec<P>;
     constexpr int N = T::N;
@@ -1036,11 +1039,35 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
     for (; k < nv; k += kBigBlock) T::add(acc, bv[k], rv[k]);
     __syncthreads();
     const int r0 = b * R, rows = min(R, m - r0);
+    if constexpr (FOLD == kFoldScan) {
+        // positions [r0, r0 + rows) hold the band's rows in another order: each row once, so
+        // these adds do not collide. A whole window: each thread loads its 16 consecutive
+        // positions with 16-byte loads, all in flight before the adds.
+        static_assert(kHybWin == 16 * kBigBlock, "scan fold: 16 positions per thread");
+        if (rows == kHybWin) {
+            const long long p0 = (long long)r0 + 16 * (long long)threadIdx.x;
+            constexpr int NV = 16 * (int)sizeof(Y) / 16;
+            float4 yv[NV];
+            uint4 mv[2];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) yv[j] = reinterpret_cast<const float4 *>(fys + p0)[j];
+            mv[0] = reinterpret_cast<const uint4 *>(fmap + p0)[0];
+            mv[1] = reinterpret_cast<const uint4 *>(fmap + p0)[1];
+            const Y *yy = reinterpret_cast<const Y *>(yv);
+            const uint16_t *mm = reinterpret_cast<const uint16_t *>(mv);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[mm[j]] += (double)yy[j];
+        } else {
+            for (int p = threadIdx.x; p < rows; p += kBigBlock)
+                acc[fmap[(long long)r0 + p]] += (double)fys[(long long)r0 + p];
+        }
+        __syncthreads();
+    }
     for (int r = threadIdx.x; r < rows; r += kBigBlock) {
         double v = acc[r];
-        if constexpr (FOLD) {
+        if constexpr (FOLD == kFoldGather) {
             const long long g = (long long)r0 + r;
-            v += (double)fys[(g & ~(long long)(kHybWin - 1)) + fiperm[g]];
+            v += (double)fys[(g & ~(long long)(kHybWin - 1)) + fmap[g]];
         }
         if constexpr (ASSIGN) y[r0 + r] = (Y)v;
         else y[r0 + r] += (Y)v;
@@ -1092,7 +1119,7 @@ static void launch_win(const fl_matrix *A, const V *val, const X *x, Y *y, bool 
 
 template <typename V, typename X, typename P, typename Y, int MODE>
 static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *y, bool assign,
-                      hipStream_t s, const Y *fys, const uint16_t *fiperm) {
+                      hipStream_t s, const PbFold *fold) {
     constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group, as in WIN
     const PackFmt pf = A->pb_pf;
     const uint16_t *lslot = MODE == kPacked ? nullptr : A->pb_lslot.p;
@@ -1103,11 +1130,17 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
               : (nt == 512) ? k_pb1<V, X, P, MODE, U, 512> : k_pb1<V, X, P, MODE, U, 1024>;
     allow_lds(k1, std::max<size_t>(lds1, 8));
     const size_t lds2 = (size_t)A->pb_R * sizeof(double);
-    allow_lds(k_pb2<P, Y, 8, true, false>, lds2);
-    allow_lds(k_pb2<P, Y, 8, false, false>, lds2);
-    if (fys) {
-        allow_lds(k_pb2<P, Y, 8, true, true>, lds2);
-        allow_lds(k_pb2<P, Y, 8, false, true>, lds2);
+    const Y *fys = fold ? (const Y *)fold->ys : nullptr;
+    const bool scan = fys && fold->perm && A->pb_R == kHybWin;
+    const uint16_t *fmap = !fys ? nullptr : scan ? fold->perm : fold->iperm;
+    allow_lds(k_pb2<P, Y, 8, true, kFoldNone>, lds2);
+    allow_lds(k_pb2<P, Y, 8, false, kFoldNone>, lds2);
+    if (scan) {
+        allow_lds(k_pb2<P, Y, 8, true, kFoldScan>, lds2);
+        allow_lds(k_pb2<P, Y, 8, false, kFoldScan>, lds2);
+    } else if (fys) {
+        allow_lds(k_pb2<P, Y, 8, true, kFoldGather>, lds2);
+        allow_lds(k_pb2<P, Y, 8, false, kFoldGather>, lds2);
     }
     // chunk by chunk: phase 1 writes the chunk's products to the front of the product array,
     // phase 2 adds them into y (the first chunk assigns y when asked to)
@@ -1116,11 +1149,15 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
         k1<<<nbc, nt, lds1, s>>>(ord_tiles(A), A->bt, b0, lslot, val, pf, x, A->pb_run_off.p,
                                  A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, bins);
         FL_LAUNCH_CHECK();
-        const bool fold = fys && c == 0;  // the first chunk's phase 2 writes every row once
-        auto k2 = (assign && c == 0) ? (fold ? k_pb2<P, Y, 8, true, true> : k_pb2<P, Y, 8, true, false>)
-                                     : (fold ? k_pb2<P, Y, 8, false, true> : k_pb2<P, Y, 8, false, false>);
+        // the first chunk's phase 2 writes every row once: it adds the folded part
+        const int fm = (fys && c == 0) ? (scan ? kFoldScan : kFoldGather) : kFoldNone;
+        auto k2 = (assign && c == 0)
+                      ? (fm == kFoldScan ? k_pb2<P, Y, 8, true, kFoldScan>
+                         : fm == kFoldGather ? k_pb2<P, Y, 8, true, kFoldGather> : k_pb2<P, Y, 8, true, kFoldNone>)
+                      : (fm == kFoldScan ? k_pb2<P, Y, 8, false, kFoldScan>
+                         : fm == kFoldGather ? k_pb2<P, Y, 8, false, kFoldGather> : k_pb2<P, Y, 8, false, kFoldNone>);
         k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p + (size_t)c * A->nbands, bins,
-                                              A->pb_chunk_off[c], A->pb_rowlocal.p, y, fys, fiperm);
+                                              A->pb_chunk_off[c], A->pb_rowlocal.p, y, fys, fmap);
         FL_LAUNCH_CHECK();
     }
 }
@@ -1129,7 +1166,7 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
 // its products to pb_bins when given (the caller guarantees pb_ring entries), else to the
 // matrix's own product array, allocated on first use.
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
-                 void *pb_bins, const void *fold_ys, const uint16_t *fold_iperm) {
+                 void *pb_bins, const PbFold *fold) {
     const bool win = (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR);
     double *bins64 = (double *)pb_bins;
     float *bins32 = (float *)pb_bins;
@@ -1142,23 +1179,23 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
     switch (kind) {
         case FL_VAL_F64:
             if (win) launch_win<double, double, double, kPlain>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
-            else launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s, (const double *)fold_ys, fold_iperm);
+            else launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_F32:
             if (win) launch_win<float, float, float, kPlain>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s, (const float *)fold_ys, fold_iperm);
+            else launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_BF16:
             if (win) launch_win<uint16_t, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s, (const float *)fold_ys, fold_iperm);
+            else launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_BF16X2:
             if (win) launch_win<uint32_t, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
-            else launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s, (const double *)fold_ys, fold_iperm);
+            else launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_PACKED:
             if (win) launch_win<uint32_t, float, float, kPacked>(A, A->win_pack.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s, (const float *)fold_ys, fold_iperm);
+            else launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s, fold);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);
