@@ -51,6 +51,33 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;  // 4 waves
 
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// A load with the nontemporal (streaming) cache hint, for data a kernel reads exactly once: the
+// matrix's value and index streams, and products written by the previous kernel. Those lines
+// are then the first to be evicted, so the L2 and the Infinity Cache keep what is reused (x, the
+// products still to be read, the heavy part's y); on the power-law workload the HYB schedule's
+// three kernels take 137 instead of 146 us with it. T is a 2-, 4-, 8- or 16-byte type.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "ld_stream: size");
+    T out;
+    if constexpr (sizeof(T) == 16) {
+        const u32x4 r = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+        __builtin_memcpy(&out, &r, 16);
+    } else if constexpr (sizeof(T) == 8) {
+        const u32x2 r = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(p));
+        __builtin_memcpy(&out, &r, 8);
+    } else if constexpr (sizeof(T) == 4) {
+        const unsigned r = __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p));
+        __builtin_memcpy(&out, &r, 4);
+    } else {
+        const unsigned short r = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p));
+        __builtin_memcpy(&out, &r, 2);
+    }
+    return out;
+}
 inline unsigned grid_for(long long threads, int block = kBlock) { return (unsigned)((threads + block - 1) / block); }
 
 // RAII device buffer (hipMalloc'd, freed on scope exit).

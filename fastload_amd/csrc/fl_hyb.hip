@@ -354,7 +354,7 @@ __device__ __forceinline__ void hyb_enter(HybCursor &c, int nslice, const long l
 // Issues the loads of the cursor's next group (always: past the last slice, or past a slice's pair
 // rows, they read padding or the next slice's slots, which hyb_consume skips) and advances the
 // cursor, entering the next slice after a slice's last group.
-template <typename V2, int U>
+template <typename V2, int U, bool S>
 __device__ __forceinline__ void hyb_load(HybGroup<V2, U> &g, HybCursor &c, int nw, int g0, int nslice,
                                          const long long *__restrict__ sb, const int *__restrict__ np,
                                          long long pad_slot, int lane, const V2 *__restrict__ val2,
@@ -366,8 +366,13 @@ __device__ __forceinline__ void hyb_load(HybGroup<V2, U> &g, HybCursor &c, int n
     const long long slot = g.valid ? c.slot + 64LL * c.q : pad_slot + lane;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        g.v[u] = val2[slot + 64 * u];
-        g.c[u] = col2[slot + 64 * u];
+        if constexpr (S) {
+            g.v[u] = ld_stream(&val2[slot + 64 * u]);
+            g.c[u] = ld_stream(&col2[slot + 64 * u]);
+        } else {
+            g.v[u] = val2[slot + 64 * u];
+            g.c[u] = col2[slot + 64 * u];
+        }
     }
     c.q += U;
     g.last = c.q >= c.P;
@@ -404,7 +409,8 @@ __device__ __forceinline__ void hyb_consume(double &acc, const HybGroup<V2, U> &
 // of the next group (possibly the next slice's first) in flight while the current one is summed.
 // Every group's loads are unconditional, since a conditional load would make the compiler wait
 // for all loads in flight before each sum.
-template <typename V2, typename X, typename Y, int U>
+// S: read the slots with the streaming hint (stream_reads).
+template <typename V2, typename X, typename Y, int U, bool S>
 __global__ __launch_bounds__(1024) void k_hyb(int m, int H, int nslice, const long long *__restrict__ sb,
                                               const int *__restrict__ np, long long pad_slot,
                                               const V2 *__restrict__ val2, const uint32_t *__restrict__ col2,
@@ -436,12 +442,12 @@ __global__ __launch_bounds__(1024) void k_hyb(int m, int H, int nslice, const lo
     hyb_enter(c, nslice, sb, np, pad_slot, lane);
     HybGroup<V2, U> a, b;
     double acc = 0.0;
-    hyb_load<V2, U>(a, c, nw, g0, nslice, sb, np, pad_slot, lane, val2, col2);
+    hyb_load<V2, U, S>(a, c, nw, g0, nslice, sb, np, pad_slot, lane, val2, col2);
     while (a.valid) {
-        hyb_load<V2, U>(b, c, nw, g0, nslice, sb, np, pad_slot, lane, val2, col2);
+        hyb_load<V2, U, S>(b, c, nw, g0, nslice, sb, np, pad_slot, lane, val2, col2);
         hyb_consume<V2, X, Y, U>(acc, a, xs, m, lane, ys);
         if (!b.valid) break;
-        hyb_load<V2, U>(a, c, nw, g0, nslice, sb, np, pad_slot, lane, val2, col2);
+        hyb_load<V2, U, S>(a, c, nw, g0, nslice, sb, np, pad_slot, lane, val2, col2);
         hyb_consume<V2, X, Y, U>(acc, b, xs, m, lane, ys);
     }
 }
@@ -499,10 +505,10 @@ size_t hyb_scratch_bytes(const fl_matrix *A, int kind) {
 }
 
 template <typename V2, typename X, typename Y>
-static void launch_hyb(const fl_matrix *A, const V2 *val2, const X *xh, Y *ys, hipStream_t s) {
+static void launch_hyb(const fl_matrix *A, const V2 *val2, const X *xh, Y *ys, bool stream, hipStream_t s) {
     constexpr int U = kHybPairs;
     const size_t lds = ((size_t)A->hyb_H + 1) * sizeof(X);
-    auto k = k_hyb<V2, X, Y, U>;
+    auto k = stream ? k_hyb<V2, X, Y, U, true> : k_hyb<V2, X, Y, U, false>;
     FL_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     // two workgroups per compute unit (32 waves) when their LDS fits, else one
     const int per_cu = lds <= 80 * 1024 ? 2 : 1;
@@ -531,9 +537,13 @@ int spmv_hyb(fl_matrix *A, int kind, const void *x, void *y, bool assign, hipStr
         }
     }
     const bool heavy = A->hyb_nnz > 0;
+    // one streaming decision for the three kernels, which share the caches
+    const long long heavy_bytes = heavy ? A->hyb_slots * (long long)(dbl ? 16 + 4 : 8 + 4) : 0;
+    const bool stream =
+        stream_reads(heavy_bytes + (L->tilenum > 0 ? batched_stream_bytes(L, FL_KERNEL_PB, kind) : 0));
     if (heavy) {
-        if (dbl) launch_hyb<double2, double, double>(A, (const double2 *)A->hyb_val64.p, (const double *)x + nl, (double *)ys, s);
-        else launch_hyb<float2, float, float>(A, (const float2 *)A->hyb_val32.p, (const float *)x + nl, (float *)ys, s);
+        if (dbl) launch_hyb<double2, double, double>(A, (const double2 *)A->hyb_val64.p, (const double *)x + nl, (double *)ys, stream, s);
+        else launch_hyb<float2, float, float>(A, (const float2 *)A->hyb_val32.p, (const float *)x + nl, (float *)ys, stream, s);
     }
     if (L->tilenum > 0) {
         PbFold fold;
@@ -542,7 +552,7 @@ int spmv_hyb(fl_matrix *A, int kind, const void *x, void *y, bool assign, hipStr
             fold.iperm = A->hyb_iperm.p;
             fold.perm = A->hyb_perm.p;
         }
-        return spmv_batched(L, FL_KERNEL_PB, kind, x, y, assign, s, bins, &fold);
+        return spmv_batched(L, FL_KERNEL_PB, kind, x, y, assign, s, bins, &fold, stream ? 1 : 0);
     }
     if (heavy) {
         if (dbl) {

@@ -181,9 +181,23 @@ struct PbFold {
     const uint16_t *perm = nullptr;
 };
 
-// The WIN/WINR/PB launches (fl_sched.hip); fold: PB only, see PbFold.
+// The WIN/WINR/PB launches (fl_sched.hip); fold: PB only, see PbFold. stream: 1 / 0 reads the
+// value, slot and product streams with / without the streaming hint, -1 decides by size
+// (stream_reads of batched_stream_bytes).
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
-                 void *pb_bins, const PbFold *fold = nullptr);
+                 void *pb_bins, const PbFold *fold = nullptr, int stream = -1);
+// Bytes a WIN/WINR/PB SpMV reads once: values (+ 16-bit slots unless packed), and for PB the
+// products and their 16-bit rows read back by phase 2.
+long long batched_stream_bytes(const fl_matrix *A, int kernel, int kind);
+
+// Infinity Cache capacity of the MI355X. A schedule that streams more than this per SpMV loads
+// its read-once streams with the streaming hint (ld_stream): they cannot stay cached between
+// calls anyway, and the hint keeps what is reused (x, products still to be read, the HYB
+// heavy part's y) from being evicted. A smaller matrix stays in the Infinity Cache from one call
+// to the next with ordinary loads (the 55.7 M-entry stencil with packed values: 0.064 ms
+// against 0.069 ms with the hint).
+constexpr long long kInfinityCacheBytes = 256LL << 20;
+inline bool stream_reads(long long bytes) { return bytes > kInfinityCacheBytes; }
 
 // HYB schedule (fl_hyb.hip). Heavy x is staged in LDS: at most kHybMaxH32 FP32 or kHybMaxH64
 // FP64 columns.

@@ -764,14 +764,16 @@ __device__ __forceinline__ WaveTiles wave_tiles(const OrdTiles &ot, int bt, int 
 
 // Value modes of the batched kernels: kPlain -- a value array (double, float or BF16) next to the
 // 16-bit slot array; kSplit -- hi+lo BF16 pair in one 32-bit word (hi << 16 | lo) next to the slot
-// array; kPacked -- value and slot in one 32-bit word (PackFmt), no slot array.
-constexpr int kPlain = 0, kSplit = 1, kPacked = 2;
+// array; kPacked -- value and slot in one 32-bit word (PackFmt), no slot array. kStream, or'ed in,
+// reads the value and slot streams with the streaming hint (ld_stream; see stream_reads).
+constexpr int kPlain = 0, kSplit = 1, kPacked = 2, kStream = 4;
+constexpr int mode_kind(int mode) { return mode & 3; }
 
 // Raw operands of U tiles of one wave (value, x, LDS slot; slot -1 = no entry), loaded one group
 // ahead of the group being accumulated. Packed words are decoded to float as they are loaded.
 template <typename V, typename X, int MODE, int U>
 struct TileGroup {
-    std::conditional_t<MODE == kPacked, float, V> v[U];
+    std::conditional_t<mode_kind(MODE) == kPacked, float, V> v[U];
     X xv[U];
     int sl[U];
 };
@@ -786,6 +788,12 @@ __device__ __forceinline__ auto widen(V v) {
 template <typename T>
 __device__ __forceinline__ T lane_load(const T *__restrict__ base, uint32_t l) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + l * (uint32_t)sizeof(T));
+}
+
+// the same for a stream read once (values, slots)
+template <typename T>
+__device__ __forceinline__ T lane_load_stream(const T *__restrict__ base, uint32_t l) {
+    return ld_stream(reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + l * (uint32_t)sizeof(T)));
 }
 
 // Tile u of the group: its entries are lanes [0, len) at entry offset ptr and sorted column cx (all
@@ -809,14 +817,15 @@ __device__ __forceinline__ void load_group(const OrdTiles &ot, const WaveTiles &
         const bool act = lane < len;
         const uint32_t l = act ? (uint32_t)lane : 0u;
         // uniform base (scalar registers) + 32-bit lane byte offset: no 64-bit address arithmetic
-        if constexpr (MODE == kPacked) {
-            const uint32_t wd = lane_load(val + ptr, l);
+        constexpr bool S = (MODE & kStream) != 0;
+        if constexpr (mode_kind(MODE) == kPacked) {
+            const uint32_t wd = S ? lane_load_stream(val + ptr, l) : lane_load(val + ptr, l);
             G.sl[u] = act ? (int)(wd & pf.imask) : dummy + lane;
             G.v[u] = pack_decode(wd, pf);
         } else {
-            const uint16_t sv = lane_load(slot + ptr, l);
+            const uint16_t sv = S ? lane_load_stream(slot + ptr, l) : lane_load(slot + ptr, l);
             G.sl[u] = act ? (int)sv : dummy + lane;
-            G.v[u] = lane_load(val + ptr, l);
+            G.v[u] = S ? lane_load_stream(val + ptr, l) : lane_load(val + ptr, l);
         }
         G.xv[u] = lane_load(x + cx, l);
     }
@@ -828,7 +837,7 @@ template <typename V, typename X, int MODE, int U>
 __device__ __forceinline__ double group_product(const TileGroup<V, X, MODE, U> &G, int u) {
     if constexpr (sizeof(X) == 8) {
         return (double)widen(G.v[u]) * G.xv[u];
-    } else if constexpr (MODE == kSplit) {  // hi+lo BF16 pair in one word: hi << 16 | lo
+    } else if constexpr (mode_kind(MODE) == kSplit) {  // hi+lo BF16 pair in one word: hi << 16 | lo
         const float hi = __uint_as_float(G.v[u] & 0xffff0000u), lo = __uint_as_float(G.v[u] << 16);
         return (double)(hi * G.xv[u]) + (double)(lo * G.xv[u]);
     } else {
@@ -991,7 +1000,7 @@ struct PbVec<double> {
 // is written; kFoldScan (bands of kHybWin rows, so band b is window b) reads the band's fys in
 // order after the products and adds fys[p] to the accumulator of row fmap[p] of the band.
 constexpr int kFoldNone = 0, kFoldGather = 1, kFoldScan = 2;
-template <typename P, typename Y, int U, bool ASSIGN, int FOLD>
+template <typename P, typename Y, int U, bool ASSIGN, int FOLD, bool S>
 __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__restrict__ band_start,
                                                    const P *__restrict__ bins, int off,
                                                    const uint16_t *__restrict__ rowlocal, Y *__restrict__ y,
@@ -1021,8 +1030,13 @@ __global__ __launch_bounds__(kBigBlock) void k_pb2(int R, int m, const int *__re
     auto load = [&](int k, typename T::V *v, typename T::R *r) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            v[u] = bv[k + u * kBigBlock];
-            r[u] = rv[k + u * kBigBlock];
+            if constexpr (S) {
+                v[u] = ld_stream(&bv[k + u * kBigBlock]);
+                r[u] = ld_stream(&rv[k + u * kBigBlock]);
+            } else {
+                v[u] = bv[k + u * kBigBlock];
+                r[u] = rv[k + u * kBigBlock];
+            }
         }
     };
     auto add = [&](const typename T::V *v, const typename T::R *r) {
@@ -1109,7 +1123,7 @@ template <typename V, typename X, typename Y, int MODE>
 static void launch_win(const fl_matrix *A, const V *val, const X *x, Y *y, bool assign,
                        hipStream_t s) {
     const PackFmt pf = A->win_pf;
-    const uint16_t *slot = MODE == kPacked ? nullptr : A->win_slot.p;
+    const uint16_t *slot = mode_kind(MODE) == kPacked ? nullptr : A->win_slot.p;
     // LDS for the most distinct rows any batch has (at most bt*64). Workgroups of 4 waves per 64
     // tiles (16 tiles per wave; 512 threads at most): more, smaller workgroups per CU overlap
     // the three phases of a batch (slot set-up, accumulation, flush) better than 1024 threads.
@@ -1131,7 +1145,8 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
                       hipStream_t s, const PbFold *fold) {
     constexpr int U = sizeof(X) == 4 ? 8 : 4;  // tiles per group, as in WIN
     const PackFmt pf = A->pb_pf;
-    const uint16_t *lslot = MODE == kPacked ? nullptr : A->pb_lslot.p;
+    const uint16_t *lslot = mode_kind(MODE) == kPacked ? nullptr : A->pb_lslot.p;
+    constexpr bool S = (MODE & kStream) != 0;
     const size_t lds1 = ((size_t)A->bt * 64 + 64) * sizeof(P);  // + one spare slot per lane
     // 4 waves per 64 tiles of the batch (16 tiles per wave)
     const int nt = A->bt * 4;
@@ -1142,14 +1157,14 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
     const Y *fys = fold ? (const Y *)fold->ys : nullptr;
     const bool scan = fys && fold->perm && A->pb_R == kHybWin;
     const uint16_t *fmap = !fys ? nullptr : scan ? fold->perm : fold->iperm;
-    allow_lds(k_pb2<P, Y, 8, true, kFoldNone>, lds2);
-    allow_lds(k_pb2<P, Y, 8, false, kFoldNone>, lds2);
+    allow_lds(k_pb2<P, Y, 8, true, kFoldNone, S>, lds2);
+    allow_lds(k_pb2<P, Y, 8, false, kFoldNone, S>, lds2);
     if (scan) {
-        allow_lds(k_pb2<P, Y, 8, true, kFoldScan>, lds2);
-        allow_lds(k_pb2<P, Y, 8, false, kFoldScan>, lds2);
+        allow_lds(k_pb2<P, Y, 8, true, kFoldScan, S>, lds2);
+        allow_lds(k_pb2<P, Y, 8, false, kFoldScan, S>, lds2);
     } else if (fys) {
-        allow_lds(k_pb2<P, Y, 8, true, kFoldGather>, lds2);
-        allow_lds(k_pb2<P, Y, 8, false, kFoldGather>, lds2);
+        allow_lds(k_pb2<P, Y, 8, true, kFoldGather, S>, lds2);
+        allow_lds(k_pb2<P, Y, 8, false, kFoldGather, S>, lds2);
     }
     // chunk by chunk: phase 1 writes the chunk's products to the front of the product array,
     // phase 2 adds them into y (the first chunk assigns y when asked to)
@@ -1161,10 +1176,10 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
         // the first chunk's phase 2 writes every row once: it adds the folded part
         const int fm = (fys && c == 0) ? (scan ? kFoldScan : kFoldGather) : kFoldNone;
         auto k2 = (assign && c == 0)
-                      ? (fm == kFoldScan ? k_pb2<P, Y, 8, true, kFoldScan>
-                         : fm == kFoldGather ? k_pb2<P, Y, 8, true, kFoldGather> : k_pb2<P, Y, 8, true, kFoldNone>)
-                      : (fm == kFoldScan ? k_pb2<P, Y, 8, false, kFoldScan>
-                         : fm == kFoldGather ? k_pb2<P, Y, 8, false, kFoldGather> : k_pb2<P, Y, 8, false, kFoldNone>);
+                      ? (fm == kFoldScan ? k_pb2<P, Y, 8, true, kFoldScan, S>
+                         : fm == kFoldGather ? k_pb2<P, Y, 8, true, kFoldGather, S> : k_pb2<P, Y, 8, true, kFoldNone, S>)
+                      : (fm == kFoldScan ? k_pb2<P, Y, 8, false, kFoldScan, S>
+                         : fm == kFoldGather ? k_pb2<P, Y, 8, false, kFoldGather, S> : k_pb2<P, Y, 8, false, kFoldNone, S>);
         k2<<<A->nbands, kBigBlock, lds2, s>>>(A->pb_R, A->m, A->pb_band_start.p + (size_t)c * A->nbands, bins,
                                               A->pb_chunk_off[c], A->pb_rowlocal.p, y, fys, fmap);
         FL_LAUNCH_CHECK();
@@ -1174,8 +1189,15 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
 // The caller (spmv) has checked that the schedule and the value copy are prepared. PB writes
 // its products to pb_bins when given (the caller guarantees pb_ring entries), else to the
 // matrix's own product array, allocated on first use.
+long long batched_stream_bytes(const fl_matrix *A, int kernel, int kind) {
+    const int v = kind == FL_VAL_F64 ? 8 : kind == FL_VAL_BF16 ? 2 : 4;  // value (packed: with its slot)
+    long long per = v + (kind == FL_VAL_PACKED ? 0 : 2);
+    if (kernel == FL_KERNEL_PB) per += (kind == FL_VAL_F64 ? 8 : 4) + 2;  // products and rows read back
+    return (long long)A->nnz * per;
+}
+
 int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, bool assign, hipStream_t s,
-                 void *pb_bins, const PbFold *fold) {
+                 void *pb_bins, const PbFold *fold, int stream) {
     const bool win = (kernel == FL_KERNEL_WIN || kernel == FL_KERNEL_WINR);
     double *bins64 = (double *)pb_bins;
     float *bins32 = (float *)pb_bins;
@@ -1185,26 +1207,37 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
         bins64 = A->pb_bins64.p;
         bins32 = A->pb_bins32.p;
     }
+    const bool st = stream < 0 ? stream_reads(batched_stream_bytes(A, kernel, kind)) : stream != 0;
     switch (kind) {
         case FL_VAL_F64:
-            if (win) launch_win<double, double, double, kPlain>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
-            else launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s, fold);
+            if (win) st ? launch_win<double, double, double, kPlain | kStream>(A, A->val64.p, (const double *)x, (double *)y, assign, s)
+                        : launch_win<double, double, double, kPlain>(A, A->val64.p, (const double *)x, (double *)y, assign, s);
+            else st ? launch_pb<double, double, double, double, kPlain | kStream>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s, fold)
+                    : launch_pb<double, double, double, double, kPlain>(A, A->val64.p, (const double *)x, bins64, (double *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_F32:
-            if (win) launch_win<float, float, float, kPlain>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s, fold);
+            if (win) st ? launch_win<float, float, float, kPlain | kStream>(A, A->val32.p, (const float *)x, (float *)y, assign, s)
+                        : launch_win<float, float, float, kPlain>(A, A->val32.p, (const float *)x, (float *)y, assign, s);
+            else st ? launch_pb<float, float, float, float, kPlain | kStream>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s, fold)
+                    : launch_pb<float, float, float, float, kPlain>(A, A->val32.p, (const float *)x, bins32, (float *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_BF16:
-            if (win) launch_win<uint16_t, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s, fold);
+            if (win) st ? launch_win<uint16_t, float, float, kPlain | kStream>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s)
+                        : launch_win<uint16_t, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, (float *)y, assign, s);
+            else st ? launch_pb<uint16_t, float, float, float, kPlain | kStream>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s, fold)
+                    : launch_pb<uint16_t, float, float, float, kPlain>(A, A->bf16_hi.p, (const float *)x, bins32, (float *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_BF16X2:
-            if (win) launch_win<uint32_t, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
-            else launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s, fold);
+            if (win) st ? launch_win<uint32_t, float, double, kSplit | kStream>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s)
+                        : launch_win<uint32_t, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, (double *)y, assign, s);
+            else st ? launch_pb<uint32_t, float, float, double, kSplit | kStream>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s, fold)
+                    : launch_pb<uint32_t, float, float, double, kSplit>(A, A->bf16_pair.p, (const float *)x, bins32, (double *)y, assign, s, fold);
             return FL_OK;
         case FL_VAL_PACKED:
-            if (win) launch_win<uint32_t, float, float, kPacked>(A, A->win_pack.p, (const float *)x, (float *)y, assign, s);
-            else launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s, fold);
+            if (win) st ? launch_win<uint32_t, float, float, kPacked | kStream>(A, A->win_pack.p, (const float *)x, (float *)y, assign, s)
+                        : launch_win<uint32_t, float, float, kPacked>(A, A->win_pack.p, (const float *)x, (float *)y, assign, s);
+            else st ? launch_pb<uint32_t, float, float, float, kPacked | kStream>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s, fold)
+                    : launch_pb<uint32_t, float, float, float, kPacked>(A, A->pb_pack.p, (const float *)x, bins32, (float *)y, assign, s, fold);
             return FL_OK;
         default:
             set_error("fl_spmv: unknown value kind %d", kind);

@@ -76,6 +76,37 @@ def test_full_size_learned_schedule(gpu, request, which):
     assert rel_err(dy.to_host().astype(np.float64), y_or, scale) <= 1e-5, sel.SCHEDULES[c].name
 
 
+def test_full_size_hyb_all_kinds(gpu, powerlaw):
+    """The HYB schedule at full size, where its three kernels read their value and index streams
+    with the streaming hint (about 490 MB per SpMV, above the 256 MB Infinity Cache): FP64 with
+    19,456 heavy columns, FP32 and packed light values with 32,768, against the FP64 oracle
+    (packed: the sub-matrix's stated rounding bound), y overwritten under assign."""
+    m, n, cp, ri, v, x = powerlaw
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    y64, s64 = ob.spmv_oracle(m, n, cp, ri, v, x)
+    M.prepare_hyb(19456, 128, 16384)
+    assert rel_err(device_y_hyb(M, x, fl.VAL_F64), y64, s64) <= 1e-12
+    M.prepare(fl.VAL_F32)
+    M.prepare(fl.VAL_PACKED)
+    M.prepare_hyb(None, 128, 16384)
+    assert M.hyb_info()[0] == 32768
+    y32, s32 = ob.spmv_oracle(m, n, cp, ri, v.astype(np.float32), x.astype(np.float32))
+    assert rel_err(device_y_hyb(M, x, fl.VAL_F32), y32, s32) <= 1e-5
+    yx, sx = ob.spmv_oracle(m, n, cp, ri, v, x.astype(np.float32).astype(np.float64))
+    mb = M.pack_info(fl.KERNEL_HYB)[0]
+    assert rel_err(device_y_hyb(M, x, fl.VAL_PACKED), yx, sx) <= 2.0 ** -(mb + 1) * (1 + 1e-6) + 1e-6
+
+
+def device_y_hyb(M, x, kind):
+    dt = np.float64 if kind == fl.VAL_F64 else np.float32
+    dx = DeviceArray.from_host(np.asarray(x, dt))
+    dsx = DeviceArray(M.n, dt)
+    M.permute_x(dx, dsx)
+    dy = DeviceArray.from_host(np.full(M.m, 9.0, dt))
+    M.spmv(dsx, dy, fl.KERNEL_HYB, kind, assign=True)
+    return dy.to_host().astype(np.float64)
+
+
 def test_full_size_checksum_and_linearity(gpu, powerlaw):
     """1'(A x) = (1'A) x, with the column sums 1'A taken from the CSC input in FP64; and
     A(x1 + 2 x2) = A x1 + 2 A x2 in FP64, for the PB and drop-in schedules."""
