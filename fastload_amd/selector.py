@@ -110,7 +110,7 @@ DATA_DIR = Path(__file__).resolve().parent / "data"
 DATASET_PATH = DATA_DIR / "selector_sweep.npz"
 # Model-guided tuning times every schedule the network gives at least TUNE_MIN_PROB probability
 # (the TUNE_MAX most likely at most, the top one always). On the training matrices 2 % gives a
-# lower regret than timing the top three (1.0001 against 1.0008) with fewer timings (2.6).
+# lower regret than timing the top three (1.0002 against 1.0006) with fewer timings (2.1).
 TUNE_MIN_PROB = 0.02
 TUNE_MAX = 5
 WEIGHTS_PATH = DATA_DIR / "selector_weights.npz"
