@@ -68,6 +68,7 @@ SIGNATURES = {
     "fl_matrix_pack_info": (C.c_int, [vp, C.c_int, i32p, i32p, i32p]),
     "fl_spmv": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
     "fl_spmv_assign": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp]),
+    "fl_set_stream_policy": (C.c_int, [C.c_int]),
     "fl_spmv_assign_noalloc": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),
     "fl_matrix_prepare_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
     "fl_matrix_prepare_hyb": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),

@@ -459,6 +459,16 @@ class Graph:
             pass
 
 
+STREAM_AUTO, STREAM_NEVER, STREAM_ALWAYS = -1, 0, 1
+
+
+def set_stream_policy(policy: int):
+    """Cache policy of the WIN/WINR/PB/HYB read-once streams (fl_set_stream_policy): STREAM_AUTO
+    (the streaming hint above the 256 MB Infinity Cache), STREAM_NEVER or STREAM_ALWAYS;
+    process-wide. y does not depend on it."""
+    check(lib.fl_set_stream_policy(int(policy)), "fl_set_stream_policy")
+
+
 def zero_device(d: DeviceArray, stream=None):
     check(lib.fl_zero_device(d.ptr, d.nbytes, stream.ptr if stream is not None else None), "fl_zero_device")
 

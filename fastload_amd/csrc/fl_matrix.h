@@ -197,7 +197,12 @@ long long batched_stream_bytes(const fl_matrix *A, int kernel, int kind);
 // to the next with ordinary loads (the 55.7 M-entry stencil with packed values: 0.064 ms
 // against 0.069 ms with the hint).
 constexpr long long kInfinityCacheBytes = 256LL << 20;
-inline bool stream_reads(long long bytes) { return bytes > kInfinityCacheBytes; }
+// fl_set_stream_policy: FL_STREAM_AUTO (by size), FL_STREAM_NEVER or FL_STREAM_ALWAYS
+int stream_policy();
+inline bool stream_reads(long long bytes) {
+    const int p = stream_policy();
+    return p == FL_STREAM_AUTO ? bytes > kInfinityCacheBytes : p == FL_STREAM_ALWAYS;
+}
 
 // HYB schedule (fl_hyb.hip). Heavy x is staged in LDS: at most kHybMaxH32 FP32 or kHybMaxH64
 // FP64 columns.

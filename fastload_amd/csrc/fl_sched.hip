@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <type_traits>
 
+#include <atomic>
+
 #include "fl_matrix.h"
 
 namespace fl {
@@ -1189,6 +1191,9 @@ static void launch_pb(const fl_matrix *A, const V *val, const X *x, P *bins, Y *
 // The caller (spmv) has checked that the schedule and the value copy are prepared. PB writes
 // its products to pb_bins when given (the caller guarantees pb_ring entries), else to the
 // matrix's own product array, allocated on first use.
+static std::atomic<int> g_stream_policy{FL_STREAM_AUTO};
+int stream_policy() { return g_stream_policy.load(std::memory_order_relaxed); }
+
 long long batched_stream_bytes(const fl_matrix *A, int kernel, int kind) {
     const int v = kind == FL_VAL_F64 ? 8 : kind == FL_VAL_BF16 ? 2 : 4;  // value (packed: with its slot)
     long long per = v + (kind == FL_VAL_PACKED ? 0 : 2);
@@ -1246,3 +1251,13 @@ int spmv_batched(fl_matrix *A, int kernel, int kind, const void *x, void *y, boo
 }
 
 }  // namespace fl
+
+extern "C" int fl_set_stream_policy(int policy) {
+    if (policy != FL_STREAM_AUTO && policy != FL_STREAM_NEVER && policy != FL_STREAM_ALWAYS) {
+        fl::set_error("fl_set_stream_policy: policy must be FL_STREAM_AUTO, FL_STREAM_NEVER or FL_STREAM_ALWAYS");
+        return FL_ERR_INVALID;
+    }
+    fl::g_stream_policy.store(policy, std::memory_order_relaxed);
+    return FL_OK;
+}
+

@@ -272,6 +272,17 @@ int fl_spmv(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d
 int fl_spmv_assign(fl_matrix *A, int kernel, int val_kind, const void *d_sortx, void *d_y, void *d_work,
                    void *stream);
 
+/* Cache policy of the read-once streams (values, slots, PB products, HYB slots) of the WIN/WINR,
+ * PB and HYB schedules; no reference counterpart. FL_STREAM_AUTO (the default): the streaming
+ * (nontemporal) hint when a schedule streams more than the 256 MB Infinity Cache per SpMV,
+ * ordinary loads otherwise; FL_STREAM_NEVER / FL_STREAM_ALWAYS force one of the two. Applies
+ * to every later call in the process; y does not depend on it, only the speed does. Returns
+ * FL_ERR_INVALID for another value. */
+#define FL_STREAM_AUTO (-1)
+#define FL_STREAM_NEVER 0
+#define FL_STREAM_ALWAYS 1
+int fl_set_stream_policy(int policy);
+
 /* fl_matrix_save writes a device-resident matrix: its layout (width, column order, tiles,
  * elements, classification), which value copies were prepared, and the prepared WIN/WINR, PB and
  * row-order schedules. fl_matrix_load reads one back into a new handle: every row and column

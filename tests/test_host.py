@@ -424,3 +424,18 @@ def test_slide_file_hashes_and_tamper_detection(tmp_path):
     with pytest.raises(fl.FastLoadError) as e:
         fl.FastLoad_cpu(t2, nnz, m, n, g["sortx"])
     assert e.value.rc == -8
+
+
+def test_stream_policy_values():
+    """fl_set_stream_policy accepts the three policies and refuses anything else (host only: it
+    sets a process-wide switch and enqueues nothing)."""
+    try:
+        for p in (fl.STREAM_NEVER, fl.STREAM_ALWAYS, fl.STREAM_AUTO):
+            assert fl.lib.fl_set_stream_policy(p) == 0
+        for bad in (2, -2, 100):
+            assert fl.lib.fl_set_stream_policy(bad) == -8
+            assert b"fl_set_stream_policy" in fl.lib.fl_last_error()
+        with pytest.raises(fl.FastLoadError):
+            fl.set_stream_policy(7)
+    finally:
+        fl.set_stream_policy(fl.STREAM_AUTO)
