@@ -108,6 +108,38 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
 };
 
+// Temporary device buffer of a preparation step, allocated and freed in the order of stream s
+// (hipMallocAsync / hipFreeAsync from the device's default memory pool): the host neither waits
+// for the device to free it nor pays a fresh hipMalloc for the next one, and the memory is
+// reused by later temporaries on the same stream after the kernels that used it. The pool keeps
+// up to kTmpPoolKeep bytes cached between calls (set once per device on first use).
+constexpr unsigned long long kTmpPoolKeep = 8ull << 30;
+void tmp_pool_init();
+template <typename T>
+struct TmpBuf {
+    T *p = nullptr;
+    size_t count = 0;
+    hipStream_t s = nullptr;
+    explicit TmpBuf(hipStream_t st) : s(st) {}
+    TmpBuf(size_t c, hipStream_t st) : s(st) { alloc(c); }
+    void alloc(size_t c) {
+        release();
+        count = c;
+        if (c) {
+            tmp_pool_init();
+            FL_HIP(hipMallocAsync((void **)&p, c * sizeof(T), s));
+        }
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        count = 0;
+    }
+    ~TmpBuf() { release(); }
+    TmpBuf(const TmpBuf &) = delete;
+    TmpBuf &operator=(const TmpBuf &) = delete;
+};
+
 // ----------------------------------------------------------------- device-side primitives
 // Exclusive prefix sum: out[0]=0, out[i]=sum(in[0..i-1]), out[count]=total. in may equal out
 // only when in has count+1 entries. Returns nothing; total is on the device at out[count].

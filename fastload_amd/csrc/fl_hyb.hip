@@ -202,15 +202,15 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
     A->hyb_ys64.release();
     const int m = A->m, n = A->n, nnz = A->nnz, T = A->tilenum, nl = n - H;
     // entries by sorted column (stable: inside a column in tile order, i.e. by depth)
-    DevBuf<uint32_t> key(std::max(nnz, 1));
-    DevBuf<int> order(std::max(nnz, 1));
+    TmpBuf<uint32_t> key(std::max(nnz, 1), s);
+    TmpBuf<int> order(std::max(nnz, 1), s);
     if (nnz > 0) {
         k_hyb_scol<<<hg((long long)T * A->W), kBlock, 0, s>>>(T, A->W, A->tile_ptr.p, A->tile_len.p, A->tile_colidx.p,
                                                              key.p, order.p);
         FL_LAUNCH_CHECK();
         radix_sort_pairs(key.p, order.p, nnz, bits_for((uint32_t)std::max(n - 1, 0)), s);
     }
-    DevBuf<int> cps((size_t)n + 1);
+    TmpBuf<int> cps((size_t)n + 1, s);
     exclusive_scan_i32(A->sortnnz.p, cps.p, n, s);
     const int nnz_l = nl > 0 ? hread(cps.p + nl, s) : 0;
     const int nnz_h = nnz - nnz_l;
@@ -230,8 +230,8 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
         L->max_col_nnz = hread(A->sortnnz.p + nl - 1, s);
     }
     {
-        DevBuf<int> trow(std::max(nnz_l, 1));
-        DevBuf<double> tval(std::max(nnz_l, 1));
+        TmpBuf<int> trow(std::max(nnz_l, 1), s);
+        TmpBuf<double> tval(std::max(nnz_l, 1), s);
         if (nnz_l > 0) {
             k_hyb_light<<<hg(nnz_l), kBlock, 0, s>>>(order.p, A->sortrowidx.p, A->val64.p, nnz_l, trow.p, tval.p);
             FL_LAUNCH_CHECK();
@@ -254,9 +254,9 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
     A->hyb_iperm.alloc(std::max(m, 1));
     A->hyb_perm.alloc(std::max(m, 1));
     long long slots = 0;
-    DevBuf<int> hptr((size_t)m + 1), rows(std::max(m, 1)), flag(1);
-    DevBuf<uint32_t> rkey(std::max(nnz_h, 1));
-    DevBuf<int> ridx(std::max(nnz_h, 1));
+    TmpBuf<int> hptr((size_t)m + 1, s), rows(std::max(m, 1), s), flag(1, s);
+    TmpBuf<uint32_t> rkey(std::max(nnz_h, 1), s);
+    TmpBuf<int> ridx(std::max(nnz_h, 1), s);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     FL_HIP(hipMemsetAsync(A->hyb_sb.p, 0, sizeof(long long) * ((size_t)nslice + 1), s));
     if (m > 0) {
@@ -268,7 +268,7 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
         k_hyb_bounds<<<hg((long long)nnz_h + 1), kBlock, 0, s>>>(rkey.p, nnz_h, m, hptr.p);
         FL_LAUNCH_CHECK();
         // rows by (window, heavy count descending, row): two stable passes
-        DevBuf<uint32_t> wkey(m);
+        TmpBuf<uint32_t> wkey(m, s);
         k_hyb_lenkey<<<hg(m), kBlock, 0, s>>>(hptr.p, m, wkey.p, rows.p);
         FL_LAUNCH_CHECK();
         radix_sort_pairs(wkey.p, rows.p, m, 16, s);
@@ -276,7 +276,7 @@ void hyb_prepare(fl_matrix *A, int H, int bt, int R, hipStream_t s) {
         FL_LAUNCH_CHECK();
         radix_sort_pairs(wkey.p, rows.p, m, bits_for((uint32_t)((m - 1) >> kHybWinLog)), s);
         k_hyb_pos<<<hg(m), kBlock, 0, s>>>(rows.p, m, A->hyb_iperm.p, A->hyb_perm.p, flag.p);
-        DevBuf<int> pairs(std::max(nslice, 1)), pscan((size_t)nslice + 1);
+        TmpBuf<int> pairs(std::max(nslice, 1), s), pscan((size_t)nslice + 1, s);
         k_hyb_pairs<<<hg(nslice), kBlock, 0, s>>>(rows.p, hptr.p, m, nslice, A->hyb_np.p, pairs.p);
         FL_LAUNCH_CHECK();
         exclusive_scan_i32(pairs.p, pscan.p, nslice, s);

@@ -60,7 +60,7 @@ __global__ void k_check_idx(const int *__restrict__ idx, int count, int limit, i
 
 // Returns 0 if ptr[0]==0, ptr[count]==nnz, ptr monotone and every idx in [0, limit).
 int check_compressed(const int *d_ptr, int count, const int *d_idx, int nnz, int limit, hipStream_t s) {
-    DevBuf<int> flag(1);
+    TmpBuf<int> flag(1, s);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     if (count > 0) {
         k_check_ptr<<<stride_grid(count), kBlock, 0, s>>>(d_ptr, count, nnz, flag.p);
@@ -105,8 +105,8 @@ void csr2csc_device(int m, int n, int nnz, const int *d_rowptr, const int *d_col
                     double *d_cscval, int *d_cscrow, int *d_colptr, int *d_nnzpercol, hipStream_t s) {
     FL_HIP(hipMemsetAsync(d_nnzpercol, 0, sizeof(int) * std::max(n, 1), s));
     if (nnz > 0) {
-        DevBuf<uint32_t> keys(nnz);
-        DevBuf<int> order(nnz), row_of(nnz);
+        TmpBuf<uint32_t> keys(nnz, s);
+        TmpBuf<int> order(nnz, s), row_of(nnz, s);
         k_copy_keys<<<stride_grid(nnz), kBlock, 0, s>>>(d_colind, keys.p, nnz);
         k_iota<<<stride_grid(nnz), kBlock, 0, s>>>(order.p, nnz);
         k_row_of<<<stride_grid(m), kBlock, 0, s>>>(d_rowptr, m, row_of.p);
@@ -123,7 +123,7 @@ void csr2csc_device(int m, int n, int nnz, const int *d_rowptr, const int *d_col
 // ------------------------------------------------------------------------ column sort
 void colsort_device(int n, const int *d_counts, int *d_perm, int *d_sortnnz, int max_key, hipStream_t s) {
     if (n <= 0) return;
-    DevBuf<uint32_t> keys(n);
+    TmpBuf<uint32_t> keys(n, s);
     k_copy_keys<<<stride_grid(n), kBlock, 0, s>>>(d_counts, keys.p, n);
     k_iota<<<stride_grid(n), kBlock, 0, s>>>(d_perm, n);
     FL_LAUNCH_CHECK();
@@ -177,7 +177,7 @@ __global__ void k_lower_bound(const int *__restrict__ sorted, int n, int key, in
 void sorted_copy(int n, const int *d_perm, const int *d_sortnnz, const int *d_P, const int *d_colptr,
                  const int *d_rowidx, const double *d_val, int *d_tmp_row, double *d_tmp_val, hipStream_t s) {
     if (n <= 0) return;
-    DevBuf<int> lf(1);
+    TmpBuf<int> lf(1, s);
     k_lower_bound<<<1, 64, 0, s>>>(d_sortnnz, n, 64, lf.p);
     FL_LAUNCH_CHECK();
     int long_from = n;
@@ -350,7 +350,7 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
     FL_HIP(hipEventRecord(e0, s));
 
     // levels
-    DevBuf<int> flags(std::max(n, 1)), lidx((size_t)n + 1);
+    TmpBuf<int> flags(std::max(n, 1), s), lidx((size_t)n + 1, s);
     if (n > 0) {
         k_level_flags<<<stride_grid(n), kBlock, 0, s>>>(d_sortnnz, n, flags.p);
         FL_LAUNCH_CHECK();
@@ -363,7 +363,7 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
     A->lvl_tall.alloc(std::max(L, 1));
     A->lvl_tileoff.alloc((size_t)L + 1);
     A->lvl_nnzoff.alloc((size_t)L + 1);
-    DevBuf<int> lvl_tiles(std::max(L, 1)), lvl_nnz(std::max(L, 1));
+    TmpBuf<int> lvl_tiles(std::max(L, 1), s), lvl_nnz(std::max(L, 1), s);
     if (n > 0) {
         k_level_fill<<<stride_grid(n), kBlock, 0, s>>>(d_sortnnz, n, W, lidx.p, A->lvl_col.p, A->lvl_base.p,
                                                        A->lvl_tall.p, lvl_tiles.p, lvl_nnz.p);
@@ -384,7 +384,7 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
     A->tile_len.alloc(std::max(T, 1));
     A->tile_colidx.alloc(std::max(T, 1));
     A->tile_format.alloc(std::max(T, 1));
-    DevBuf<int> tile_depth(std::max(T, 1));
+    TmpBuf<int> tile_depth(std::max(T, 1), s);
     if (!src.tiled) {
         A->sortrowidx.alloc(std::max(nnz, 1));
         A->val64.alloc(std::max(nnz, 1));
@@ -407,7 +407,7 @@ void transform_device(fl_matrix *A, const int *d_sortnnz, const ElemSource &src,
     FL_HIP(hipEventRecord(e1, s));
 
     // classification
-    DevBuf<int> seg_count(std::max(T, 1));
+    TmpBuf<int> seg_count(std::max(T, 1), s);
     A->countrow.alloc((size_t)T + 1);
     if (T > 0) {
         const long long waves = ((long long)T + (64 / W) - 1) / (64 / W);

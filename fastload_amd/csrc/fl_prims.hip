@@ -1,6 +1,7 @@
 // Device primitives for the preprocessing pipeline: exclusive scan, stable LSD radix sort of
 // (key, index) pairs, max reduction. These replace the reference's CUB DeviceScan calls
 // (src/formatTransform_GPU.h:402-496) and thrust::sort_by_key (src/ColSort_GPU.h:101).
+#include <atomic>
 #include <cstdarg>
 #include <vector>
 
@@ -78,7 +79,7 @@ void exclusive_scan_i32(const int *in, int *out, int count, hipStream_t s) {
         FL_LAUNCH_CHECK();
         return;
     }
-    DevBuf<int> partial(nb), partial_scan((size_t)nb + 1);
+    TmpBuf<int> partial(nb, s), partial_scan((size_t)nb + 1, s);
     k_tile_sums<<<nb, kBlock, 0, s>>>(in, count, partial.p);
     FL_LAUNCH_CHECK();
     exclusive_scan_i32(partial.p, partial_scan.p, nb, s);
@@ -193,9 +194,9 @@ void radix_sort_pairs(uint32_t *keys, int *vals, int count, int bits, hipStream_
         set_error("radix sort: histogram too large");
         throw HipError{hipErrorInvalidValue};
     }
-    DevBuf<int> hist((size_t)hcount), offs((size_t)hcount + 1);
-    DevBuf<uint32_t> k2(count);
-    DevBuf<int> v2(count);
+    TmpBuf<int> hist((size_t)hcount, s), offs((size_t)hcount + 1, s);
+    TmpBuf<uint32_t> k2(count, s);
+    TmpBuf<int> v2(count, s);
     uint32_t *kin = keys, *kout = k2.p;
     int *vin = vals, *vout = v2.p;
     const int passes = (bits + kRadixBits - 1) / kRadixBits;
@@ -213,7 +214,23 @@ void radix_sort_pairs(uint32_t *keys, int *vals, int count, int bits, hipStream_
         FL_HIP(hipMemcpyAsync(keys, kin, sizeof(uint32_t) * count, hipMemcpyDeviceToDevice, s));
         FL_HIP(hipMemcpyAsync(vals, vin, sizeof(int) * count, hipMemcpyDeviceToDevice, s));
     }
-    FL_HIP(hipStreamSynchronize(s));
+    // no wait here: the temporaries are freed in stream order, and the caller's later work on s
+    // (or its own readback) comes after the sort
+}
+
+// ------------------------------------------------------------------ temporaries' memory pool
+void tmp_pool_init() {
+    static std::atomic<unsigned long long> done{0};  // one bit per device (up to 64)
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    const unsigned long long bit = 1ull << dev;
+    if (done.load(std::memory_order_acquire) & bit) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = kTmpPoolKeep;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    done.fetch_or(bit, std::memory_order_acq_rel);
 }
 
 // ------------------------------------------------------------------------------- errors

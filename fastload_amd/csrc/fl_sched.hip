@@ -288,8 +288,8 @@ static void ensure_batches(fl_matrix *A, int bt, int kind, hipStream_t s) {
         A->ord_kind = kind;
         A->bt = 0;  // batches follow the tile order: rebuild them
         if (T > 0) {
-            DevBuf<uint32_t> key(T);
-            DevBuf<int> order(T);
+            TmpBuf<uint32_t> key(T, s);
+            TmpBuf<int> order(T, s);
             if (kind == 1) k_tile_rowkey<<<sg(T), kBlock, 0, s>>>(A->tile_ptr.p, A->sortrowidx.p, T, key.p);
             else k_tile_bucket<<<sg(T), kBlock, 0, s>>>(A->tile_colidx.p, T, key.p);
             k_iota2<<<sg(T), kBlock, 0, s>>>(order.p, T);
@@ -310,7 +310,7 @@ static void ensure_batches(fl_matrix *A, int bt, int kind, hipStream_t s) {
     A->pb_pack.release();
     A->batch_of.alloc(std::max(A->nnz, 1));
     A->batch_start.alloc((size_t)A->nbatches + 1);
-    DevBuf<int> sizes(std::max(A->nbatches, 1));
+    TmpBuf<int> sizes(std::max(A->nbatches, 1), s);
     if (T > 0) {
         k_batch_of<<<sg((long long)T * 64), kBlock, 0, s>>>(T, bt, A->n, A->ord_ptr.p, A->ord_cx.p, A->batch_of.p);
         k_batch_sizes<<<sg(A->nbatches), kBlock, 0, s>>>(T, bt, A->nbatches, A->n, A->ord_cx.p, sizes.p);
@@ -336,8 +336,8 @@ void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
         A->win_ready = true;
         return;
     }
-    DevBuf<uint32_t> key(nnz);
-    DevBuf<int> order(nnz), head(nnz), d((size_t)nnz + 1);
+    TmpBuf<uint32_t> key(nnz, s);
+    TmpBuf<int> order(nnz, s), head(nnz, s), d((size_t)nnz + 1, s);
     // (row, g) order, then stable by batch -> (batch, row, g)
     k_key_copy<<<sg(nnz), kBlock, 0, s>>>(A->sortrowidx.p, key.p, nnz, 1);
     k_iota2<<<sg(nnz), kBlock, 0, s>>>(order.p, nnz);
@@ -357,7 +357,7 @@ void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
     k_win_slot<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, A->sortrowidx.p, head.p, d.p, A->win_off.p, A->win_slot.p,
                                           A->win_rows.p, nnz);
     FL_LAUNCH_CHECK();
-    DevBuf<int> flag(1);
+    TmpBuf<int> flag(1, s);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     k_check_win<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->win_slot.p, A->win_off.p, bt * 64, nnz, A->m,
                                            A->win_rows.p, flag.p);
@@ -367,14 +367,14 @@ void win_prepare(fl_matrix *A, int bt, bool rows, hipStream_t s) {
         throw HipError{hipErrorInvalidValue};
     }
     {
-        DevBuf<int> cnt(std::max(A->m, 1));
+        TmpBuf<int> cnt(std::max(A->m, 1), s);
         FL_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * std::max(A->m, 1), s));
         k_row_batches<<<sg(D), kBlock, 0, s>>>(A->win_rows.p, D, cnt.p);
         k_mark_shared<<<sg(D), kBlock, 0, s>>>(A->win_rows.p, D, cnt.p);
         FL_LAUNCH_CHECK();
         FL_HIP(hipStreamSynchronize(s));
     }
-    DevBuf<int> per_batch(nb);
+    TmpBuf<int> per_batch(nb, s);
     k_adjdiff<<<sg(nb), kBlock, 0, s>>>(A->win_off.p, nb, per_batch.p);
     FL_LAUNCH_CHECK();
     A->win_maxslots = device_max_i32(per_batch.p, nb, s);
@@ -443,10 +443,10 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
         throw HipError{hipErrorInvalidValue};
     }
     A->pb_band_start.alloc((size_t)C * nbands + 1);
-    DevBuf<int> d_cb(cb.size()), d_off(C);
+    TmpBuf<int> d_cb(cb.size(), s), d_off(C, s);
     FL_HIP(hipMemcpyAsync(d_cb.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice, s));
-    DevBuf<uint32_t> key(nnz);
-    DevBuf<int> order(nnz), dest(nnz), head(nnz), runidx((size_t)nnz + 1);
+    TmpBuf<uint32_t> key(nnz, s);
+    TmpBuf<int> order(nnz, s), dest(nnz, s), head(nnz, s), runidx((size_t)nnz + 1, s);
     // A: chunk-major, then band-major order (chunk, band, batch, g): stable by batch (batches in
     // XCD order), then stable by (chunk, band), so each (batch, band) run occupies one contiguous
     // range of the product array
@@ -483,14 +483,14 @@ void pb_prepare(fl_matrix *A, int bt, int R, hipStream_t s) {
     A->pb_run_lstart.alloc(nruns);
     A->pb_run_dest.alloc(nruns);
     A->pb_run_len.alloc(nruns);
-    DevBuf<int> run_pos(nruns);
+    TmpBuf<int> run_pos(nruns, s);
     k_pb_runs<<<sg(nnz), kBlock, 0, s>>>(key.p, order.p, head.p, runidx.p, dest.p, A->batch_start.p, nbands,
                                          A->pb_lslot.p, A->pb_run_lstart.p, A->pb_run_dest.p, run_pos.p,
                                          A->pb_run_off.p, nnz);
     k_run_len<<<sg(nruns), kBlock, 0, s>>>(run_pos.p, nruns, nnz, A->pb_run_len.p);
     FL_LAUNCH_CHECK();
     write1(A->pb_run_off.p + nb, nruns, s);
-    DevBuf<int> flag(1);
+    TmpBuf<int> flag(1, s);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->batch_start.p, A->bt * 64, nnz, ring,
                                           A->pb_run_off.p, A->pb_run_lstart.p, A->pb_run_dest.p, A->pb_run_len.p, nb,
@@ -588,7 +588,7 @@ static bool pack_format(const fl_matrix *A, int ib, PackFmt &f) {
 void pack_schedules(fl_matrix *A, hipStream_t s) {
     const int nnz = A->nnz;
     if (!A->pack_range_known) {
-        DevBuf<int> r(3);
+        TmpBuf<int> r(3, s);
         const int init[3] = {0x7fffffff, -0x7fffffff, 0};
         FL_HIP(hipMemcpyAsync(r.p, init, sizeof init, hipMemcpyHostToDevice, s));
         if (nnz) k_exp_range<<<std::min(sg(nnz), 4096u), kBlock, 0, s>>>(A->val64.p, nnz, r.p);
@@ -657,7 +657,7 @@ int win_install(fl_matrix *A, int bt, bool rows, int nb_file, const uint16_t *sl
     upload_to(A->win_slot, slot, nnz, s);
     upload_to(A->win_rows, win_rows, (size_t)D, s);
     upload_to(A->win_off, off, (size_t)nb + 1, s);
-    DevBuf<int> flag(1);
+    TmpBuf<int> flag(1, s);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     if (nnz)
         k_check_win<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->win_slot.p, A->win_off.p, bt * 64, nnz, A->m,
@@ -708,7 +708,7 @@ int pb_install(fl_matrix *A, int bt, int R, int nb_file, int C, const int *chunk
     upload_to(A->pb_run_lstart, run_lstart, (size_t)nruns, s);
     upload_to(A->pb_run_dest, run_dest, (size_t)nruns, s);
     upload_to(A->pb_run_len, run_len, (size_t)nruns, s);
-    DevBuf<int> flag(1);
+    TmpBuf<int> flag(1, s);
     FL_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     if (nnz)
         k_check_pb<<<sg(nnz), kBlock, 0, s>>>(A->batch_of.p, A->pb_lslot.p, A->batch_start.p, bt * 64, nnz, (int)ring,
