@@ -120,7 +120,8 @@ int bits_for(uint32_t v) {
 // One 8-bit LSD pass = histogram per block -> scan (digit-major) -> stable scatter.
 // Block tile: kSortRounds rounds of 256 keys in index order; inside a round the rank of a key
 // among equal digits is (earlier waves' counts) + (lower lanes of its wave with that digit),
-// the latter from a ballot-based match over the 8 digit bits.
+// the latter from a ballot-based match over the 8 digit bits. The scatter stages the tile in
+// LDS in digit order before writing it out.
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortRounds = 16;
@@ -140,19 +141,46 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restric
     hist[(long long)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// The tile's keys are first placed in LDS in (digit, input order) order, each at its digit's
+// local offset plus its stable rank; then they are written out from LDS position i to
+// global_base[d] + (i - local_base[d]): consecutive threads write consecutive positions of one
+// digit's run, instead of every key going straight to one of 256 scattered buckets. The tile's
+// digit counts are the differences of the scanned histogram (offsets).
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__restrict__ keys_in,
                                                           const int *__restrict__ vals_in,
                                                           uint32_t *__restrict__ keys_out, int *__restrict__ vals_out,
                                                           int count, int shift, int nblocks,
                                                           const int *__restrict__ offsets) {
-    __shared__ int base[kRadix];
+    static_assert(kRadix == kBlock, "one digit per thread");
+    __shared__ uint32_t skey[kSortTile];
+    __shared__ int sval[kSortTile];
+    __shared__ int lbase[kRadix], gbase[kRadix], run[kRadix];
+    __shared__ int wsum[kBlock / kWave];
     __shared__ int wcnt[kBlock / kWave][kRadix];  // tagged with the round: (round << 8) | count
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    base[threadIdx.x] = offsets[(long long)threadIdx.x * nblocks + blockIdx.x];
+    {
+        const long long idx = (long long)threadIdx.x * nblocks + blockIdx.x;
+        const int g = offsets[idx];
+        const int c = offsets[idx + 1] - g;  // keys of digit threadIdx.x in this tile
+        gbase[threadIdx.x] = g;
+        // exclusive scan of the 256 counts: inside each wave, then across the 4 waves
+        int v = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o, 64);
+            if (lane >= o) v += u;
+        }
+        if (lane == 63) wsum[w] = v;
+        __syncthreads();
+        int before = 0;
+        for (int q = 0; q < w; ++q) before += wsum[q];
+        lbase[threadIdx.x] = before + v - c;
+        run[threadIdx.x] = before + v - c;
+    }
     for (int q = 0; q < kBlock / kWave; ++q) wcnt[q][threadIdx.x] = -1;
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const long long tile = (long long)blockIdx.x * kSortTile;
+    const int n_tile = (int)min((long long)kSortTile, (long long)count - tile);
     for (int r = 0; r < kSortRounds; ++r) {
         const long long i = tile + (long long)r * kBlock + threadIdx.x;
         const bool valid = i < count;
@@ -172,17 +200,24 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t *__rest
         if (leader) wcnt[w][d] = (r << 8) | cnt;
         __syncthreads();
         if (valid) {
-            int pos = base[d] + rank;
+            int pos = run[d] + rank;
             for (int q = 0; q < w; ++q) {
                 const int e = wcnt[q][d];
                 pos += ((e >> 8) == r) ? (e & 0xff) : 0;
             }
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            skey[pos] = key;
+            sval[pos] = val;
         }
         __syncthreads();
-        if (leader) atomicAdd(&base[d], cnt);
+        if (leader) atomicAdd(&run[d], cnt);
         __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n_tile; i += kBlock) {
+        const uint32_t key = skey[i];
+        const int d = (int)((key >> shift) & (kRadix - 1));
+        const int g = gbase[d] + (i - lbase[d]);
+        keys_out[g] = key;
+        vals_out[g] = sval[i];
     }
 }
 
