@@ -15,8 +15,8 @@ Variants (schedules x value storage):
   winr*    the same kernel on tiles grouped by row window (most rows owned by one batch)
   pb_*     two-phase propagation blocking over 16,384-row bands, FP64 LDS sums, no global atomics
   row_*    entries re-ordered by row, four lanes per row, no atomics
-  hyb_*    the min(n, 32768) longest columns row by row with their x staged in LDS, the other
-           columns through PB on a sub-matrix
+  hyb_*    the min(n, 32768) longest columns (19,456 in FP64) row by row with their x staged in
+           LDS, the other columns through PB on a sub-matrix
   learned_* the schedule the learned selector picks for the matrix: features and MLP inference
            on the GPU (committed trained weights) rank the 13 schedules, those of probability
            >= 2 % (at most 5) are timed on the matrix and the fastest is used; in each value
@@ -77,9 +77,11 @@ VARIANTS = {
     "pb8k_f32": (64, 3, 1, 4, 256, 8192),
     "pb4k_f32": (64, 3, 1, 4, 256, 4096),
     "pb128_8k_f32": (64, 3, 1, 4, 128, 8192),
-    # HYB (kernel 5): heaviest 32768 columns from LDS (FP32 values), the rest PB 128 x 16K
+    # HYB (kernel 5): heaviest 32768 columns from LDS (FP32 values), the rest PB 128 x 16K; in
+    # FP64 the heaviest 19456 (x takes 8 bytes in LDS)
     "hyb_f32": (64, 5, 1, 4, 128, 16384),
     "hyb_packed": (64, 5, 4, 4, 128, 16384),
+    "hyb_f64": (64, 5, 0, 8, 128, 16384),
 }
 # value storage -> (name, value bytes beside a 4-byte row index); packed words hold the value and
 # the schedule's slot in 4 bytes, so they count 0 + 4
@@ -209,7 +211,7 @@ def run_workload(fl, name, m, n, dcp, dri, dv, x, variants, steps, warmup, flush
             if kernel in (fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_PB):
                 M.prepare_schedule(kernel, bt, max(R, 1024))
             elif kernel == fl.KERNEL_HYB:
-                M.prepare_hyb(None, bt, R)
+                M.prepare_hyb(min(n, 19456) if kind == fl.VAL_F64 else None, bt, R)
             ydev = y64 if kind in (fl.VAL_F64, fl.VAL_BF16X2) else y32
             assign = kernel in (fl.KERNEL_PB, fl.KERNEL_WIN, fl.KERNEL_WINR, fl.KERNEL_HYB)
 
@@ -716,7 +718,7 @@ def main():
     ap.add_argument("--batch-k", default="1,8,32", help="right-hand-side counts of the batched configuration")
     ap.add_argument("--no-seq", action="store_true", help="skip the batched configuration's sequential baselines")
     ap.add_argument("--no-powerlaw", action="store_true", help="skip the power-law workload (profiling runs)")
-    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32,hyb_f32")
+    ap.add_argument("--variants", default="plain_f32,plain_f64,fast_f32,win_f32,pb_f32,pb_f64,row_f32,hyb_f32,hyb_f64")
     ap.add_argument("--stencil-variants", default="plain_f32,fast_f32,win_f32,win_f64,winr64_f32,pb_f32,row_f32")
     ap.add_argument("--schedule", default=None, help="use this schedule for the learned variants instead of "
                     "selecting one (profiling runs)")
