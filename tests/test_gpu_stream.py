@@ -56,3 +56,32 @@ def test_stream_policy_does_not_change_y(gpu, kernel):
             assert rel_err(y1.astype(np.float64), y_or, scale) <= tol, (kernel, kind)
     finally:
         fl.set_stream_policy(fl.STREAM_AUTO)
+
+
+def test_preparation_on_two_user_streams(gpu):
+    """Layout, PB and HYB preparation of two matrices enqueued on two streams of the caller, one
+    after the other without waiting: their temporaries come from the stream-ordered pool, so each
+    must only reuse memory its own stream has finished with. Both y agree with the oracle."""
+    streams = [fl.Stream(), fl.Stream()]
+    specs = [(71, 60000, 50000), (72, 80000, 65000)]
+    mats, data = [], []
+    for (seed, m0, n0), s in zip(specs, streams):
+        m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, seed, m0, n0, (1.8, 5000))
+        M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64, stream=s)
+        M.prepare(fl.VAL_F32, stream=s)
+        M.prepare_schedule(fl.KERNEL_PB, 128, 8192, stream=s)
+        mats.append(M)
+        data.append((m, n, cp, ri, v))
+    for M, s in zip(mats, streams):
+        M.prepare_hyb(4096, 128, 16384, stream=s)
+    for (m, n, cp, ri, v), M, s in zip(data, mats, streams):
+        x = np.random.default_rng(m).standard_normal(n).astype(np.float32)
+        dx = DeviceArray.from_host(x)
+        dsx = DeviceArray(n, np.float32)
+        M.permute_x(dx, dsx, stream=s)
+        y_or, scale = ob.spmv_oracle(m, n, cp, ri, v.astype(np.float32), x)
+        for kernel in (fl.KERNEL_PB, fl.KERNEL_HYB):
+            dy = DeviceArray(m, np.float32)
+            M.spmv(dsx, dy, kernel, fl.VAL_F32, stream=s, assign=True)
+            s.synchronize()
+            assert rel_err(dy.to_host().astype(np.float64), y_or, scale) <= 1e-5, kernel
