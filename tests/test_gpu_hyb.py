@@ -103,6 +103,20 @@ def test_hyb_shapes(gpu, case):
     _check(m, n, cp, ri, v, x, M, kinds, case)
 
 
+@pytest.mark.parametrize("R", [16384, 8192])
+def test_hyb_fold_with_partial_last_band(gpu, R):
+    """m = 70,001 rows: with 16,384-row bands phase 2 reads the heavy part's y in sorted order
+    (whole windows with 16-byte loads, the partial last window element by element); with 8,192
+    it gathers through the row-to-position map. Both against the FP64 oracle."""
+    m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, 41, 70001, 52000, (1.9, 5000))
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    M.prepare(fl.VAL_F32)
+    M.prepare(fl.VAL_PACKED)
+    M.prepare_hyb(4096, 128, R)
+    x = np.random.default_rng(R).standard_normal(n)
+    _check(m, n, cp, ri, v, x, M, (fl.VAL_F64, fl.VAL_F32, fl.VAL_PACKED), f"R={R}")
+
+
 def test_hyb_matches_pb_and_accumulates(gpu):
     m, n, cp, ri, v = fl.gen_matrix(fl.GEN_ZIPF, 36, 120000, 120000, (1.8, 10000))
     M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
