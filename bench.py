@@ -25,7 +25,8 @@ Variants (schedules x value storage):
 `value` = GB/s-equivalent (algorithmic bytes with FP32 values / step time) of the fastest
 learned_* variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported
 beside it. The stencil configuration (128^3 27-point, 55.7M nnz) is timed too and reported
-under "stencil" with the error of every value storage.
+under "stencil" with the error of every value storage. "cg": conjugate gradients in FP64 on the
+resident 27-point SPD matrix on 128^3 (iterations to 1e-8, ms and GB/s per iteration).
 
 Algorithmic bytes per SpMV: B = nnz*(s_val + 4) + n*s_x + m*s_y.
 """
@@ -446,6 +447,64 @@ def run_batched(fl, count, seed, steps, warmup, sequential=True, ks=(1, 8, 32)):
     return res
 
 
+def run_cg(fl, steps, warmup, g=128, iters=200):
+    """Conjugate gradients (fl_cg_*) on the resident matrix, FP64: the symmetric positive definite
+    27-point matrix on g^3 (26 on the diagonal, -1 elsewhere; the stencil workload's structure)
+    with the stencil's schedule (WINR, 64-tile batches). One solve to ||r||/||b|| <= 1e-8 from
+    x = 0 (iterations, time, residual recomputed on the host), then `iters` iterations with
+    rtol = 0, timed on the host around the solve after a warm-up solve (the graph of 50
+    iterations is captured by the first solve and replayed; the host reads the residual between
+    replays). Bytes per iteration: the SpMV's algorithmic bytes in FP64 plus 13 vector passes of
+    n doubles (p permuted to sorted order: 2; p.q: 2; x += alpha p, r -= alpha q: 6; p = r + beta
+    p: 3). The SpMV alone is timed with the same schedule for comparison."""
+    import scipy.sparse as sp
+    from fastload_amd import DeviceArray
+    from fastload_amd import workloads as wl
+    m, n, cp, ri, v, b = wl.laplacian27(g)
+    nnz = int(cp[n])
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    M.prepare_schedule(fl.KERNEL_WINR, 64, 0)
+    cg = fl.CG(M, fl.KERNEL_WINR, fl.VAL_F64)
+    db = DeviceArray.from_host(b)
+    dx = DeviceArray.zeros(n, np.float64)
+    every = 50
+    t0 = time.perf_counter()
+    it_conv, res_conv = cg.solve(db, dx, max_iter=5000, rtol=1e-8, check_every=every)
+    conv_s = time.perf_counter() - t0
+    A = sp.csc_matrix((v, ri, cp), shape=(m, n))
+    x = dx.to_host()
+    true_res = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+    del A, x
+    for _ in range(max(warmup, 1)):
+        dx = DeviceArray.zeros(n, np.float64)
+        cg.solve(db, dx, max_iter=iters, rtol=0.0, check_every=every)
+    fl.synchronize()
+    dx = DeviceArray.zeros(n, np.float64)
+    t0 = time.perf_counter()
+    it, _ = cg.solve(db, dx, max_iter=iters, rtol=0.0, check_every=every)
+    fl.synchronize()
+    ms_iter = (time.perf_counter() - t0) * 1e3 / max(it, 1)
+    # the SpMV alone: y = A sortx with the same schedule
+    dsx, dy = DeviceArray.zeros(n, np.float64), DeviceArray.zeros(m, np.float64)
+    M.permute_x(DeviceArray.from_host(b), dsx)
+    spmv_step, spmv_kernel, _ = time_variant(
+        fl, lambda: M.spmv(dsx, dy, fl.KERNEL_WINR, fl.VAL_F64, assign=True), dy, steps, warmup, zero=False)
+    spmv_bytes = b_alg(nnz, n, m, 8, 8, 8)
+    it_bytes = spmv_bytes + 13 * n * 8
+    cg.close()
+    r = {"workload": f"27-point SPD matrix on {g}^3 (26 on the diagonal, -1 elsewhere), b N(0,1), FP64",
+         "n": n, "nnz": nnz, "schedule": "winr64 (WINR, 64-tile batches)", "graph_iterations": every,
+         "converged_iterations": it_conv, "converged_rel_res": res_conv, "true_rel_res": true_res,
+         "converge_s": round(conv_s, 4), "timed_iterations": it, "ms_per_iter": round(ms_iter, 5),
+         "bytes_per_iter": it_bytes, "gbs": round(it_bytes / ms_iter / 1e6, 1),
+         "frac_of_peak": round(it_bytes / ms_iter / 1e6 / PEAK_GBS, 4),
+         "spmv_step_ms": round(spmv_step, 5), "spmv_kernel_ms": round(spmv_kernel, 5),
+         "spmv_share": round(spmv_step / ms_iter, 3)}
+    log(f"[cg] {g}^3 FP64: {it_conv} iterations to 1e-8 in {conv_s:.3f} s (true res {true_res:.2e}); "
+        f"{ms_iter:.4f} ms/iteration = {r['gbs']} GB/s; SpMV alone {spmv_step:.4f} ms")
+    return r
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -691,6 +750,8 @@ def run_all(args):
         out["comparators"] = comparators
     if not args.no_c1:
         out["c1"] = run_c1(fl, args.steps, args.warmup)
+    if not args.no_cg:
+        out["cg"] = run_cg(fl, args.steps, args.warmup)
     if args.batch > 0:
         out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup, sequential=not args.no_seq,
                                      ks=tuple(int(k) for k in args.batch_k.split(",")))
@@ -712,6 +773,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-cg", action="store_true", help="skip the conjugate-gradient solve")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timings")
     ap.add_argument("--no-compare", action="store_true", help="skip the naive-CSC and rocSPARSE yardsticks")
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")

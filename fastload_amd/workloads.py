@@ -8,6 +8,9 @@ library's own generators, include/fastload_gen.h):
   powerlaw(n, seed)     n x n, column counts Zipf(1.8423) truncated at 50,000 (mean 16), rows
                         uniform without repeats per column, values N(0,1); x N(0,1)
   stencil(g)            27-point stencil on a g^3 grid, |v| U[0.5,1.5] with random sign
+  laplacian27(g)        the same 27-point structure with 26 on the diagonal and -1 elsewhere:
+                        27 I - T(x)T(x)T with T = tridiag(1, 1, 1), whose eigenvalues lie in
+                        (0, 54), so the matrix is symmetric positive definite (a CG system)
   batch(count, seed)    `count` square matrices, n log-uniform in [1e3, 2e5], column counts
                         Zipf(2.0) truncated at n (mean ~7.8), values N(0,1)
 """
@@ -33,6 +36,15 @@ def powerlaw(n: int = 4194304, seed: int = 42):
 
 def stencil(g: int = 128, seed: int = 128):
     m, n, cp, ri, v = gen_matrix(GEN_STENCIL27, seed, 0, 0, (g,))
+    return m, n, cp, ri, v, gen_vector(VEC_NORMAL, seed + 1, n)
+
+
+def laplacian27(g: int = 128, seed: int = 128):
+    """(m, n, colptr, rowidx, val, b): the symmetric positive definite 27-point matrix on g^3 and
+    a right-hand side N(0,1)."""
+    m, n, cp, ri, _ = gen_matrix(GEN_STENCIL27, seed, 0, 0, (g,))
+    col = np.repeat(np.arange(n, dtype=np.int32), np.diff(cp))
+    v = np.where(ri == col, 26.0, -1.0)
     return m, n, cp, ri, v, gen_vector(VEC_NORMAL, seed + 1, n)
 
 

@@ -439,3 +439,18 @@ def test_stream_policy_values():
             fl.set_stream_policy(7)
     finally:
         fl.set_stream_policy(fl.STREAM_AUTO)
+
+
+@pytest.mark.parametrize("g", [1, 2, 5])
+def test_laplacian27_is_the_spd_kronecker_form(g):
+    """The CG workload: 27 I - T (x) T (x) T with T = tridiag(1, 1, 1), symmetric, eigenvalues in
+    (0, 54) (those of T lie in (-1, 3), so the product's lie in (-27, 27))."""
+    import scipy.sparse as sp
+    from fastload_amd import workloads as wl
+    m, n, cp, ri, v, b = wl.laplacian27(g)
+    assert m == n == g ** 3 and len(b) == n and cp[n] == (3 * g - 2) ** 3
+    A = sp.csc_matrix((v, ri, cp), shape=(m, n)).toarray()
+    T = np.eye(g) + np.eye(g, k=1) + np.eye(g, k=-1)
+    assert np.array_equal(A, 27 * np.eye(n) - np.kron(np.kron(T, T), T))
+    ev = np.linalg.eigvalsh(A)
+    assert ev[0] > 0 and ev[-1] < 54
