@@ -26,7 +26,8 @@ Variants (schedules x value storage):
 learned_* variant whose error vs the FP64 path is <= 1e-4; the plain FP32 drop-in is reported
 beside it. The stencil configuration (128^3 27-point, 55.7M nnz) is timed too and reported
 under "stencil" with the error of every value storage. "cg": conjugate gradients in FP64 on the
-resident 27-point SPD matrix on 128^3 (iterations to 1e-8, ms and GB/s per iteration).
+resident 27-point SPD matrix on 128^3 (iterations to 1e-8, ms and GB/s per iteration);
+"cache_file": saving and loading that matrix's layout and schedule against preparing them.
 
 Algorithmic bytes per SpMV: B = nnz*(s_val + 4) + n*s_x + m*s_y.
 """
@@ -505,6 +506,58 @@ def run_cg(fl, steps, warmup, g=128, iters=200):
     return r
 
 
+def run_cache(fl, g=128):
+    """The preprocessing cache file (fl_matrix_save / fl_matrix_load) against the preparation it
+    replaces, on the 27-point SPD matrix on g^3 with the WINR schedule (64-tile batches): wall
+    time of building the device layout from host CSC arrays plus the schedule, of saving, and of
+    loading (twice: the second read finds the file in the page cache), and how far the loaded
+    matrix's y is from the original's, beside the spread of two runs of the original (WINR in
+    FP64 flushes rows that several batches touch with atomics, so y is reproducible only up to
+    the order of those additions)."""
+    import tempfile
+    from fastload_amd import DeviceArray
+    from fastload_amd import workloads as wl
+    m, n, cp, ri, v, b = wl.laplacian27(g)
+    fl.synchronize()
+    t0 = time.perf_counter()
+    M = fl.FastLoadMatrix(m, n, cp, ri, v, width=64)
+    M.prepare_schedule(fl.KERNEL_WINR, 64, 0)
+    fl.synchronize()
+    prep_s = time.perf_counter() - t0
+    fd, path = tempfile.mkstemp(suffix=".flm")
+    os.close(fd)
+    try:
+        t0 = time.perf_counter()
+        M.save(path)
+        save_s = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        loads = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            L = fl.FastLoadMatrix.load(path)
+            fl.synchronize()
+            loads.append(time.perf_counter() - t0)
+    finally:
+        os.remove(path)
+    dsx = DeviceArray.zeros(n, np.float64)
+    M.permute_x(DeviceArray.from_host(b), dsx)
+    ys = []
+    for X in (M, M, L):
+        dy = DeviceArray.zeros(m, np.float64)
+        X.spmv(dsx, dy, fl.KERNEL_WINR, fl.VAL_F64, assign=True)
+        ys.append(dy.to_host())
+    scale = max(float(np.abs(ys[0]).max()), np.finfo(np.float64).tiny)
+    r = {"workload": f"27-point SPD matrix on {g}^3, nnz {int(cp[n])}, layout W=64 + WINR schedule (64-tile batches)",
+         "prepare_s": round(prep_s, 4), "save_s": round(save_s, 4), "file_bytes": size,
+         "load_s_first": round(loads[0], 4), "load_s_cached": round(loads[1], 4),
+         "loaded_vs_original_max_diff": float(np.abs(ys[2] - ys[0]).max() / scale),
+         "original_run_to_run_max_diff": float(np.abs(ys[1] - ys[0]).max() / scale)}
+    log(f"[cache] prepare {prep_s:.3f} s, save {save_s:.3f} s ({size / 1e9:.2f} GB), load {loads[0]:.3f} s "
+        f"/ {loads[1]:.3f} s, y diff {r['loaded_vs_original_max_diff']:.1e} "
+        f"(run to run {r['original_run_to_run_max_diff']:.1e})")
+    return r
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -752,6 +805,7 @@ def run_all(args):
         out["c1"] = run_c1(fl, args.steps, args.warmup)
     if not args.no_cg:
         out["cg"] = run_cg(fl, args.steps, args.warmup)
+        out["cache_file"] = run_cache(fl)
     if args.batch > 0:
         out["batched"] = run_batched(fl, args.batch, 512, args.steps, args.warmup, sequential=not args.no_seq,
                                      ks=tuple(int(k) for k in args.batch_k.split(",")))
@@ -773,7 +827,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
-    ap.add_argument("--no-cg", action="store_true", help="skip the conjugate-gradient solve")
+    ap.add_argument("--no-cg", action="store_true", help="skip the conjugate-gradient solve and the cache file")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timings")
     ap.add_argument("--no-compare", action="store_true", help="skip the naive-CSC and rocSPARSE yardsticks")
     ap.add_argument("--batch", type=int, default=512, help="matrices in the batched configuration (0 = skip)")
