@@ -115,19 +115,30 @@ def profiled_traffic(schedule: str, kind: str, workload: str):
 
 
 def time_variant(fl, run, dy, steps, warmup, zero=True, flush=None):
-    """Step = (y = 0) + run(); events bracket every run() (the SpMV launches) and the whole loop.
-    With `flush` (a device buffer larger than the 256 MB Infinity Cache), a fill kernel over it
-    runs before every rep, outside the run() events, so no rep finds the previous rep's data in
-    the cache; the loop time then includes the fills and only the per-run times are meaningful."""
+    """Step = (y = 0) + run(). The step time comes from a loop of `steps` steps bracketed by two
+    events and nothing else; a second loop of `steps` steps records events around every run()
+    (the SpMV launches) for the kernel times. With `flush` (a device buffer larger than the
+    256 MB Infinity Cache), a fill kernel over it runs before every rep of the second loop,
+    outside the run() events, so no rep finds the previous rep's data in the cache; only the
+    per-run times are meaningful then, and the step loop is skipped (step time None)."""
     from fastload_amd import Event
     for _ in range(warmup):
         if zero:
             fl.zero_device(dy)
         run()
     fl.synchronize()
+    step_ms = None
+    if flush is None:
+        t0, t1 = Event(), Event()
+        t0.record()
+        for _ in range(steps):
+            if zero:
+                fl.zero_device(dy)
+            run()
+        t1.record()
+        fl.synchronize()
+        step_ms = t0.elapsed_ms(t1) / steps
     ev = [(Event(), Event()) for _ in range(steps)]
-    t0, t1 = Event(), Event()
-    t0.record()
     for a, b in ev:
         if flush is not None:
             fl.zero_device(flush)
@@ -136,9 +147,7 @@ def time_variant(fl, run, dy, steps, warmup, zero=True, flush=None):
         a.record()
         run()
         b.record()
-    t1.record()
     fl.synchronize()
-    step_ms = t0.elapsed_ms(t1) / steps
     kern = np.array([a.elapsed_ms(b) for a, b in ev])
     return step_ms, float(kern.mean()), float(np.median(kern))
 
